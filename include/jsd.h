@@ -1,0 +1,107 @@
+/*
+ * jsd.h - C ABI of the MI355X channel-flow step library (libjsd.so).
+ *
+ * The library advances the velocity-vorticity channel-flow equations of
+ * dakling/jax-spectral-dns by its RK3/Crank-Nicolson step on one GPU, with
+ * hand-written HIP kernels for gfx950.  All arrays handed over at run time
+ * are device pointers in the reference's layout: complex128 (interleaved
+ * re/im doubles) of shape (Nx_modes, Ny, Nz_modes) per velocity component,
+ * components contiguous, wavenumbers stored [0..K, -K..-1], y index 0 at
+ * y = +1.  Plain pointers and sizes only; no torch types cross this ABI.
+ *
+ * Every function returns JSD_OK (0) or a negative jsd_status; the message of
+ * the last error of the calling thread is available from jsd_last_error().
+ * A plan is immutable after creation and may be used by one stream at a
+ * time.  Citations are reference file:line (jax_spectral_dns package).
+ */
+#ifndef JSD_H
+#define JSD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum jsd_status {
+  JSD_OK = 0,
+  JSD_ERR_ARG = -1,    /* invalid argument or unsupported shape */
+  JSD_ERR_HIP = -2,    /* a HIP runtime call failed */
+  JSD_ERR_NOMEM = -3,  /* device allocation failed */
+  JSD_ERR_STATE = -4   /* call not valid for this plan kind */
+};
+
+enum jsd_kind {
+  JSD_KIND_NS = 0,            /* NavierStokesVelVort, skew-symmetric NL term */
+  JSD_KIND_PERTURBATION = 1,  /* NavierStokesVelVortPerturbation */
+  JSD_KIND_DUAL = 2           /* NavierStokesVelVortPerturbationDual */
+};
+
+typedef struct jsd_plan jsd_plan;
+
+/* Host-side description; every array is copied at plan creation.  The
+ * operator tables come from the host precompute (one entry per distinct
+ * (|kx|, |kz|) pair, Ka = nxm/2+1, Kb = nzm/2+1, row-major). */
+typedef struct jsd_plan_desc {
+  int nxm, ny, nzm;   /* stored modes (odd, = 1 mod 4) and Chebyshev points (<= 144) */
+  int px, pz;         /* the reference's physical grid: fixes the spectral scaling */
+  int ppx, ppz;       /* transform grid of the FFT kernels: >= 3K+1, radices 2,3,4,5,7,8 */
+  double lx, lz;      /* box lengths */
+  double re_tau;      /* Reynolds number the tables were built with (negative for the dual) */
+  double dt;          /* explicit-term time step (negative for the dual) */
+  int kind;           /* jsd_kind */
+  int linearise;      /* perturbation / dual: drop the u.grad u term (navier_stokes_perturbation.py:191) */
+  const double* D;          /* ny*ny collocation derivative matrix */
+  const double* kx;         /* nxm stored wavenumbers */
+  const double* kz;         /* nzm */
+  const double* lhs_inv;    /* 3*Ka*Kb*ny*ny inverse of hom_dir(I - beta_s dt L(k)) */
+  const double* poisson;    /* Ka*Kb*ny*ny Poisson inverse */
+  const double* v_a;        /* 3*Ka*Kb*ny wall-correction solution */
+  const double* infl_inv;   /* 3*Ka*Kb*4 inverse influence matrix */
+  const double* rhs_base;   /* 3*ny*ny hom_dir(I + alpha_s dt D^2 / Re) */
+  const double* rhs_ns;     /* 3*(2ny)^2 mean-flow block operators */
+  const double* lhs_inv_ns; /* 3*(2ny)^2 */
+  const double* base_u;     /* ny base-flow profile U(y) (perturbation / dual) */
+  const double* base_du;    /* ny U'(y) */
+  const double* y_weights;  /* ny definite-integral weights */
+} jsd_plan_desc;
+
+/* Replaces NavierStokesVelVort.__init__ / assemble_matrices
+ * (navier_stokes.py:313-384, navier_stokes.py:437-502): uploads the operator
+ * tables and allocates the step's workspaces. */
+int jsd_plan_create(const jsd_plan_desc* desc, jsd_plan** out);
+int jsd_plan_destroy(jsd_plan* plan);
+
+/* Replaces the nonlinear_update_fn plug-in
+ * (navier_stokes.py:324-332 -> :236-261; navier_stokes_perturbation.py:507-520
+ * -> :160-239; navier_stokes_perturbation_dual.py:531-540 -> :214-294):
+ * vel_hat (3 components) -> h_v, h_g (1 component each) and conv_ns
+ * (3 components).  fwd_hat is the forward state for a nonlinear dual plan,
+ * NULL otherwise. */
+int jsd_nonlinear(jsd_plan* plan, const void* vel_hat, const void* fwd_hat,
+                  void* h_v, void* h_g, void* conv_ns, void* stream);
+
+/* Replaces perform_time_step / perform_runge_kutta_step
+ * (navier_stokes.py:1778-1813, navier_stokes.py:1219-1775): advances vel_hat
+ * in place by nsteps RK3/CN steps with constant pressure gradient dPdx. */
+int jsd_step(jsd_plan* plan, void* vel_hat, double dPdx, int nsteps, void* stream);
+
+/* One dual (adjoint) step of a JSD_KIND_DUAL plan
+ * (navier_stokes_perturbation_dual.py:690-839 inner step): advances adj_hat in
+ * place; fwd_hat is the forward state history[-1-t]
+ * (navier_stokes_perturbation_dual.py:591-602), may be NULL when linearised. */
+int jsd_dual_step(jsd_plan* plan, void* adj_hat, const void* fwd_hat, void* stream);
+
+/* Replaces VectorField.energy of no_hat(vel_hat)
+ * (field.py:607-611, field.py:1468-1473): writes the energy to *out (host). */
+int jsd_energy(jsd_plan* plan, const void* vel_hat, double* out, void* stream);
+
+/* y-derivative of order 1 or 2 of ncomp spectral components
+ * (FourierDomain.diff along y, domain.py:903-917). */
+int jsd_ydiff(jsd_plan* plan, const void* in, void* out, int ncomp, int order, void* stream);
+
+const char* jsd_last_error(void);
+int jsd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JSD_H */

@@ -1,0 +1,1155 @@
+// jsd.hip - MI355X (gfx950) FP64 velocity-vorticity channel-flow step.
+//
+// One RK3/CN substep is a short pipeline of kernels over the spectral state
+// (kx, y, kz layout, complex128):
+//   1. k_ygemm<PLAIN>  : y-derivatives D u_c on FP64 MFMA (16x16x4 tiles).
+//   2a. nonlinear FFT path (NS, nonlinear perturbation):
+//        k_fft_x_inv  : Hermitian part + zero-pad + inverse x-FFT of two
+//                       packed real fields per launch,
+//        k_fft_z_mid  : inverse z-FFT of all packed fields, skew-symmetric
+//                       products in physical space, forward z-FFT,
+//        k_fft_x_fwd  : forward x-FFT, unpacking of the real pair,
+//                       truncation with the reference's +-K convention.
+//   2b. linearised path: k_nl_lin computes the same H spectrum without any
+//        FFT (products with a y-only base flow are diagonal in (kx, kz)).
+//   3. k_ygemm<HV>     : h_v = -D(ikx H0 + ikz H2) - k^2 H1, h_g, conv lines.
+//   4. k_solve         : per-(|kx|,|kz|) implicit Helmholtz/Poisson solves,
+//                        influence-matrix wall correction, u/w from
+//                        continuity, (0,0) mean-flow block.
+// Reference: jax_spectral_dns navier_stokes.py:1219-1775 (step),
+// navier_stokes.py:48-261 and navier_stokes_perturbation*.py (NL terms),
+// domain.py:716-764 / 1167-1242 (transforms).
+#include "jsd_kernels.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/jsd.h"
+
+// ===========================================================================
+// y-derivative GEMM on FP64 MFMA
+// ===========================================================================
+// For one kx row i the field is an (N x 2Z) real matrix X (rows y, columns
+// kz re/im interleaved); out = D^o X.  Tiles: 16 rows (output y) x 16 columns
+// per MFMA; the wave owns one 16-column strip and all row tiles; the WG
+// (4 waves) owns 64 columns.  D^o is pre-tiled on the host into MFMA A
+// fragments: DF[(lt*KS + ks)*64 + lane] = D^o[lt*16 + lane%16][ks*4 + lane/16].
+enum { GEMM_PLAIN = 0, GEMM_HV = 1 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, int NT, int KS,
+                                               const double* __restrict__ in0,  // PLAIN: input; HV: H0
+                                               const double* __restrict__ in1,  // HV: H1
+                                               const double* __restrict__ in2,  // HV: H2
+                                               double* __restrict__ out0,       // PLAIN: out; HV: h_v
+                                               double* __restrict__ out1,       // HV: h_g
+                                               double* __restrict__ conv,       // HV: conv lines (2*N complex)
+                                               const double* __restrict__ kx, const double* __restrict__ kz, int M,
+                                               int N, int Z, long comp_stride) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int i = blockIdx.x;
+  const int comp = blockIdx.z;
+  const int C2 = 2 * Z;
+  const int col = blockIdx.y * 64 + wave * 16 + (lane & 15);
+  const bool colok = col < C2;
+  const int kr = lane >> 4;
+  const size_t rowbase = (size_t)comp * comp_stride + (size_t)i * N * C2;
+  const double kxv = kx[i];
+  const int kzi = col >> 1;
+  const double kzv = colok ? kz[kzi] : 0.0;
+  const int par = col & 1;
+
+  dbl4 acc[JSD_NTMAX];
+#pragma unroll
+  for (int t = 0; t < JSD_NTMAX; ++t) acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+
+  auto loadB = [&](int ks) -> double {
+    const int j = ks * 4 + kr;
+    if (!colok || j >= N) return 0.0;
+    const size_t e = rowbase + (size_t)j * C2;
+    if (MODE == GEMM_PLAIN) return in0[e + col];
+    // X = (i kx) H0 + (i kz) H2 ; (i k)(a + ib) = (-k b, k a)
+    const size_t c0 = e + (kzi << 1);
+    if (par == 0) return (-kxv * in0[c0 + 1]) + (-kzv * in2[c0 + 1]);
+    return (kxv * in0[c0]) + (kzv * in2[c0]);
+  };
+
+  double bnext = loadB(0);
+  for (int ks = 0; ks < KS; ++ks) {
+    const double b = bnext;
+    if (ks + 1 < KS) bnext = loadB(ks + 1);
+    const double* A = DF + (size_t)ks * 64 + lane;
+#pragma unroll
+    for (int lt = 0; lt < JSD_NTMAX; ++lt) {
+      if (lt < NT) {
+        const double a = A[(size_t)lt * KS * 64];
+        acc[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[lt], 0, 0, 0);
+      }
+    }
+  }
+  if (!colok) return;
+#pragma unroll
+  for (int lt = 0; lt < JSD_NTMAX; ++lt) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int l = lt * 16 + r * 4 + kr;  // f64 16x16x4: reg r of lane L is row 4r + L/16
+      if (lt >= NT || l >= N) continue;
+      const size_t e = rowbase + (size_t)l * C2;
+      if (MODE == GEMM_PLAIN) {
+        out0[e + col] = acc[lt][r];
+      } else {
+        // h_v = (-D X + (-kx^2) H1) + (-kz^2) H1   (navier_stokes.py:53-61)
+        const double h1 = in1[e + col];
+        out0[e + col] = (-acc[lt][r] + (-(kxv * kxv)) * h1) + (-(kzv * kzv)) * h1;
+        // h_g = (i kz) H0 - (i kx) H2               (navier_stokes.py:62-64)
+        const size_t c0 = e + (kzi << 1);
+        double hg;
+        if (par == 0) hg = (-kzv * in0[c0 + 1]) - (-kxv * in2[c0 + 1]);
+        else hg = (kzv * in0[c0]) - (kxv * in2[c0]);
+        out1[e + col] = hg;
+        // conv_ns = -H on the (0,0) line for the mean-flow solve
+        if (i == 0 && kzi == 0) {
+          conv[2 * l + par] = -in0[e + col];
+          conv[2 * (N + l) + par] = -in2[e + col];
+        }
+      }
+    }
+  }
+}
+
+// ===========================================================================
+// FFT passes (nonlinear path)
+// ===========================================================================
+// A WG of the x-passes owns QB kz "pairs" q = q0..q0+QB-1 (q <= KZ); slot c
+// (0..2QB-1) holds column j = q (c < QB) or j = Z - q (c >= QB, q >= 1).
+// The partner column -kz of slot c is slot c ^ QB (slot c itself for q = 0).
+#define QB 8
+#define NSLOT (2 * QB)
+
+__device__ __forceinline__ int slot_col(int c, int q0, int Z, int KZ, bool* ok) {
+  const int q = q0 + (c % QB);
+  const bool neg = c >= QB;
+  *ok = (q <= KZ) && (!neg || q >= 1);
+  return neg ? Z - q : q;
+}
+__device__ __forceinline__ int slot_partner(int c, int q0) {
+  const int q = q0 + (c % QB);
+  return q == 0 ? c : (c ^ QB);
+}
+
+enum { OP_ID = 0, OP_DX = 1, OP_DZ = 2 };
+
+__device__ __forceinline__ double2 apply_op(int op, double2 v, double kxv, double kzv) {
+  if (op == OP_DX) return cik(kxv, v);
+  if (op == OP_DZ) return cik(kzv, v);
+  return v;
+}
+
+// Inverse x pass: T[x][l][j] = sum_kx (Herm(opA A) + i Herm(opB B))[kx, l, j] e^{+i kx x}
+// scaled by s_inv.  grid (nqb, N), block 256.
+__global__ __launch_bounds__(256) void k_fft_x_inv(const double2* __restrict__ A, int opA,
+                                                   const double2* __restrict__ B, int opB,
+                                                   double2* __restrict__ T, FftPlan PX, const double* __restrict__ kx,
+                                                   const double* __restrict__ kz, int M, int N, int Z,
+                                                   double s_inv) {
+  extern __shared__ double2 lds[];
+  double2* buf = lds;  // [p][slot]
+  const int q0 = blockIdx.x * QB;
+  const int l = blockIdx.y;
+  const int K = M / 2, KZ = Z / 2;
+  const int P = PX.p;
+  for (int w = threadIdx.x; w < P * NSLOT; w += blockDim.x) {
+    const int c = w % NSLOT;
+    const int pos = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    double2 val = make_double2(0.0, 0.0);
+    const int n = PX.perm[pos];  // bin index
+    int i = -1;
+    if (n <= K) i = n;
+    else if (n >= P - K) i = n - P + M;
+    if (ok && i >= 0) {
+      const int pi = (M - i) % M;
+      const int pj = (Z - j) % Z;
+      const size_t e = ((size_t)i * N + l) * Z + j;
+      const size_t ep = ((size_t)pi * N + l) * Z + pj;
+      const double2 a = apply_op(opA, A[e], kx[i], kz[j]);
+      const double2 ap = apply_op(opA, A[ep], kx[pi], kz[pj]);
+      const double2 b = apply_op(opB, B[e], kx[i], kz[j]);
+      const double2 bp = apply_op(opB, B[ep], kx[pi], kz[pj]);
+      const double2 ha = cherm(a, ap);
+      const double2 hb = cherm(b, bp);
+      val = make_double2((ha.x - hb.y) * s_inv, (ha.y + hb.x) * s_inv);
+    }
+    buf[pos * NSLOT + c] = val;
+  }
+  __syncthreads();
+  fft_lds(buf, NSLOT, NSLOT, 1, PX, +1.0);
+  for (int w = threadIdx.x; w < P * NSLOT; w += blockDim.x) {
+    const int c = w % NSLOT;
+    const int x = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    if (ok) T[((size_t)x * N + l) * Z + j] = buf[x * NSLOT + c];
+  }
+}
+
+// Physical-space product variants
+enum { PROD_NS = 0, PROD_PERT = 1, PROD_DUAL = 2 };
+
+// Middle z pass: inverse z-FFT of nt packed inputs, products, forward z-FFT of
+// two packed outputs.  grid (PXlen, N), block 256.
+// Input rows (NS / perturbation): row 2c = (u_c, d_x u_c), row 2c+1 = (d_y u_c, d_z u_c).
+// Dual adds rows 6+2c, 7+2c for the transport field W = c u_fwd (+ U on the host side).
+__global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T, long tstride, int nt,
+                                                   double2* __restrict__ V, long vstride, FftPlan PZ, int N, int Z,
+                                                   int prod, double clin, const double* __restrict__ Ub,
+                                                   const double* __restrict__ dUb) {
+  extern __shared__ double2 lds[];
+  const int P = PZ.p;
+  double2* buf = lds;             // [p][nt]
+  double2* obuf = lds + P * nt;   // [p][2]
+  const int x = blockIdx.x;
+  const int l = blockIdx.y;
+  const int KZ = Z / 2;
+  for (int w = threadIdx.x; w < P * nt; w += blockDim.x) {
+    const int t = w % nt;
+    const int pos = w / nt;
+    const int n = PZ.perm[pos];
+    int j = -1;
+    if (n <= KZ) j = n;
+    else if (n >= P - KZ) j = n - P + Z;
+    buf[pos * nt + t] = (j >= 0) ? T[t * tstride + ((size_t)x * N + l) * Z + j] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_lds(buf, nt, nt, 1, PZ, +1.0);
+  const double U = Ub ? Ub[l] : 0.0;
+  const double dU = dUb ? dUb[l] : 0.0;
+  for (int z = threadIdx.x; z < P; z += blockDim.x) {
+    double u[3], g[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2 r0 = buf[z * nt + 2 * c];
+      const double2 r1 = buf[z * nt + 2 * c + 1];
+      u[c] = r0.x;
+      g[c][0] = r0.y;
+      g[c][1] = r1.x;
+      g[c][2] = r1.y;
+    }
+    const double div = g[0][0] + g[1][1] + g[2][2];
+    double H[3];
+    if (prod == PROD_NS) {
+      // H = -(u.grad) u + 0.5 (div u) u   (navier_stokes.py:236-261)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
+        H[c] = -conv + 0.5 * (div * u[c]);
+      }
+    } else if (prod == PROD_PERT) {
+      // H = -(c u.grad u + U d_x u + v U' e_x) + 0.5 (c div u u + div u U e_x)
+      // (navier_stokes_perturbation.py:160-239)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
+        double hel = clin * conv + U * g[c][0];
+        double dv = clin * (div * u[c]);
+        if (c == 0) {
+          hel += u[1] * dU;
+          dv += div * U;
+        }
+        H[c] = -hel + 0.5 * dv;
+      }
+    } else {
+      // dual: v (rows 0..5) adjoint, W (rows 6..11) transport field (without U)
+      double wv[3], gw[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double2 r0 = buf[z * nt + 6 + 2 * c];
+        const double2 r1 = buf[z * nt + 7 + 2 * c];
+        wv[c] = clin * r0.x;
+        gw[c][0] = clin * r0.y;
+        gw[c][1] = clin * r1.x;
+        gw[c][2] = clin * r1.y;
+      }
+      wv[0] += U;
+      gw[0][1] += dU;
+      const double divw = gw[0][0] + gw[1][1] + gw[2][2];
+      // H = -(W.grad) v + [sum_j v_j d_i W_j] + 0.25 ((div W) v + (div v) W)
+      // (navier_stokes_perturbation_dual.py:214-294)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double t1 = wv[0] * g[c][0] + wv[1] * g[c][1] + wv[2] * g[c][2];
+        const double t2 = u[0] * gw[0][c] + u[1] * gw[1][c] + u[2] * gw[2][c];
+        H[c] = -t1 + t2 + 0.5 * (0.5 * (divw * u[c] + div * wv[c]));
+      }
+    }
+    // packed outputs at natural index z; DIT order applied after the barrier
+    lds[P * nt + P * 2 + z * 2 + 0] = make_double2(H[0], H[1]);
+    lds[P * nt + P * 2 + z * 2 + 1] = make_double2(H[2], 0.0);
+  }
+  __syncthreads();
+  double2* nat = lds + P * nt + P * 2;
+  for (int w = threadIdx.x; w < P * 2; w += blockDim.x) {
+    const int o = w % 2;
+    const int pos = w / 2;
+    obuf[pos * 2 + o] = nat[PZ.perm[pos] * 2 + o];
+  }
+  __syncthreads();
+  fft_lds(obuf, 2, 2, 1, PZ, -1.0);
+  for (int w = threadIdx.x; w < Z * 2; w += blockDim.x) {
+    const int o = w / Z;
+    const int j = w % Z;
+    const int n = (j <= KZ) ? j : P - (Z - j);
+    V[o * vstride + ((size_t)x * N + l) * Z + j] = obuf[n * 2 + o];
+  }
+}
+
+// Forward x pass with unpacking and the reference's +-K storage convention
+// (domain.py:724-742).  grid (nqb, N, 2), block 256.  Output t=0 writes
+// H0, H1; t=1 writes H2.
+__global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V, long vstride,
+                                                   double2* __restrict__ H, long hstride, FftPlan PX, int M, int N,
+                                                   int Z, double s_fwd) {
+  extern __shared__ double2 lds[];
+  double2* buf = lds;
+  const int q0 = blockIdx.x * QB;
+  const int l = blockIdx.y;
+  const int t = blockIdx.z;
+  const int K = M / 2, KZ = Z / 2;
+  const int P = PX.p;
+  const double2* Vt = V + t * vstride;
+  for (int w = threadIdx.x; w < P * NSLOT; w += blockDim.x) {
+    const int c = w % NSLOT;
+    const int pos = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    const int xi = PX.perm[pos];
+    buf[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * Z + j] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_lds(buf, NSLOT, NSLOT, 1, PX, -1.0);
+  auto bin = [&](int i) { return i <= K ? i : P - (M - i); };
+  for (int w = threadIdx.x; w < M * NSLOT; w += blockDim.x) {
+    const int c = w % NSLOT;
+    const int i = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    if (!ok) continue;
+    int si = i, sc = c;
+    if (i == K + 1 && j == KZ + 1) {
+      si = K;
+      sc = slot_partner(c, q0);
+    } else if (i == K + 1) {
+      sc = slot_partner(c, q0);
+    } else if (j == KZ + 1) {
+      si = (M - i) % M;
+    }
+    const int spi = (M - si) % M;
+    const double2 Wv = buf[bin(si) * NSLOT + sc];
+    const double2 Wp = buf[bin(spi) * NSLOT + slot_partner(sc, q0)];
+    const size_t e = ((size_t)i * N + l) * Z + j;
+    if (t == 0) {
+      const double2 g0 = make_double2(0.5 * (Wv.x + Wp.x), 0.5 * (Wv.y - Wp.y));
+      const double dr = Wv.x - Wp.x, di = Wv.y + Wp.y;  // W - conj(Wp)
+      const double2 g1 = make_double2(0.5 * di, -0.5 * dr);
+      H[e] = cscale(g0, s_fwd);
+      H[hstride + e] = cscale(g1, s_fwd);
+    } else {
+      const double2 g2 = make_double2(0.5 * (Wv.x + Wp.x), 0.5 * (Wv.y - Wp.y));
+      H[2 * hstride + e] = cscale(g2, s_fwd);
+    }
+  }
+}
+
+// ===========================================================================
+// linearised nonlinear term without transforms
+// ===========================================================================
+// The physical field of a stored spectrum X is IFFT(Herm(X)); multiplying by a
+// y-only profile commutes with the x-z transforms, and the forward transform
+// of the product stores G[kx*sx, kz*sz] (sz = -1 on the kx = -K row, sx = -1
+// on the kz = -K column).  Perturbation (navier_stokes_perturbation.py:160-239
+// with linearise): G0 = -(U ikx u + U' v) + 0.5 U div, G1 = -U ikx v,
+// G2 = -U ikx w.  Dual (navier_stokes_perturbation_dual.py:214-294 with
+// linearise): G0 = -U ikx v0 + 0.25 U div v, G1 = -U ikx v1 + U' v0,
+// G2 = -U ikx v2.
+__global__ __launch_bounds__(256) void k_nl_lin(const double2* __restrict__ U0, long cs, const double2* __restrict__ DV,
+                                                double2* __restrict__ H, const double* __restrict__ kx,
+                                                const double* __restrict__ kz, const double* __restrict__ Ub,
+                                                const double* __restrict__ dUb, int M, int N, int Z, int dual) {
+  const size_t tot = (size_t)M * N * Z;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= tot) return;
+  const int j = e % Z;
+  const int l = (e / Z) % N;
+  const int i = e / ((size_t)Z * N);
+  const int K = M / 2, KZ = Z / 2;
+  int si = i, sj = j;
+  if (i == K + 1 && j == KZ + 1) {
+    si = K;
+    sj = KZ;
+  } else if (i == K + 1) {
+    sj = (Z - j) % Z;
+  } else if (j == KZ + 1) {
+    si = (M - i) % M;
+  }
+  const int pi = (M - si) % M, pj = (Z - sj) % Z;
+  const size_t s0 = ((size_t)si * N + l) * Z + sj;
+  const size_t p0 = ((size_t)pi * N + l) * Z + pj;
+  const double2 hu = cherm(U0[s0], U0[p0]);
+  const double2 hv = cherm(U0[cs + s0], U0[cs + p0]);
+  const double2 hw = cherm(U0[2 * cs + s0], U0[2 * cs + p0]);
+  const double2 hdv = cherm(DV[s0], DV[p0]);
+  const double kxv = kx[si], kzv = kz[sj];
+  const double U = Ub[l], dU = dUb[l];
+  const double2 dxu = cik(kxv, hu), dxv = cik(kxv, hv), dxw = cik(kxv, hw);
+  const double2 div = cadd(cadd(dxu, hdv), cik(kzv, hw));
+  double2 g0, g1, g2;
+  if (!dual) {
+    g0 = cadd(cscale(cadd(cscale(dxu, U), cscale(hv, dU)), -1.0), cscale(div, 0.5 * U));
+    g1 = cscale(dxv, -U);
+    g2 = cscale(dxw, -U);
+  } else {
+    g0 = cadd(cscale(dxu, -U), cscale(div, 0.25 * U));
+    g1 = cadd(cscale(dxv, -U), cscale(hu, dU));
+    g2 = cscale(dxw, -U);
+  }
+  H[e] = g0;
+  H[cs + e] = g1;
+  H[2 * cs + e] = g2;
+}
+
+// ===========================================================================
+// per-wavenumber implicit solve (navier_stokes.py:1248-1546)
+// ===========================================================================
+// One WG per distinct (|kx|, |kz|) = (a, b); it handles the up-to-four signed
+// pencils (+-a, +-b).  Vectors live in LDS as [vec][y], vec = 2*pencil + re/im.
+// Matrices are stored transposed (AT[j*N + i] = A[i][j]) so consecutive
+// threads (rows i) read consecutive addresses.
+struct SolveArgs {
+  double2* U;  // state, 3 components (in place)
+  long cs;
+  const double2* HV;
+  const double2* HG;
+  const double2* HVo;
+  const double2* HGo;
+  const double2* CONV;   // (0,0) line of conv_ns: [0..N) comp 0, [N..2N) comp 2
+  const double2* CONVo;
+  const double* D_T;     // N*N transposed
+  const double* D2_T;
+  const double* RB_T;    // rhs_base[s] transposed
+  const double* LI_T;    // lhs_inv[s] transposed, per |k|
+  const double* PO_T;    // poisson transposed, per |k|
+  const double* VA;      // v_a[s], per |k|
+  const double* INFL;    // infl_inv[s], per |k|
+  const double* RNS;     // rhs_ns[s] (2N)^2 row-major
+  const double* LNS;     // lhs_inv_ns[s]
+  const double* kx;
+  const double* kz;
+  int M, N, Z, Ka, Kb;
+  double dt_gamma, dt_xi, diag_alpha;  // dt*gamma_s, dt*xi_s, alpha_s*dt/Re
+  double dpdx;                         // scaled pressure gradient
+  int use_old;
+};
+
+// Y[v][i] = sum_j A[i][j] X[v][j], rows i in [r0, r1); nv vectors
+__device__ void mv_lds(const double* __restrict__ AT, int N, const double* X, double* Y, int nv, int r0, int r1) {
+  const int nr = r1 - r0;
+  const int half = (nv + 1) / 2;
+  for (int it = threadIdx.x; it < 2 * nr; it += blockDim.x) {
+    const int i = r0 + (it % nr);
+    const int h = it / nr;
+    const int v0 = h * half;
+    const int v1 = min(nv, v0 + half);
+    double acc[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
+    for (int j = 0; j < N; ++j) {
+      const double a = AT[(size_t)j * N + i];
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        if (v0 + v < v1) acc[v] += a * X[(v0 + v) * JSD_NYMAX + j];
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+      if (v0 + v < v1) Y[(v0 + v) * JSD_NYMAX + i] = acc[v];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
+  __shared__ double sv[8 * JSD_NYMAX];    // v
+  __shared__ double sw[8 * JSD_NYMAX];    // omega_y
+  __shared__ double sphi[16 * JSD_NYMAX]; // [phi | omega] then rhs
+  __shared__ double srhs[16 * JSD_NYMAX];
+  __shared__ double snl[16 * JSD_NYMAX];  // dt*gamma*h + dt*xi*h_old  for [v | omega]
+  __shared__ double sab[4 * 4];           // a, b per pencil (complex)
+  __shared__ int pis[4], pjs[4];
+  __shared__ int npen;
+
+  const int nblk = gridDim.x;
+  // XCD-aware order: blocks dispatched round-robin over 8 XCDs; consecutive
+  // logical groups (neighbouring kz) land on the same XCD and share L2 lines.
+  const int per = (nblk + 7) / 8;
+  const int g = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (g >= S.Ka * S.Kb) return;
+  const int a = g / S.Kb, b = g % S.Kb;
+  const int M = S.M, N = S.N, Z = S.Z;
+  if (threadIdx.x == 0) {
+    int n = 0;
+    pis[n] = a; pjs[n] = b; ++n;
+    if (a > 0) { pis[n] = M - a; pjs[n] = b; ++n; }
+    if (b > 0) { pis[n] = a; pjs[n] = Z - b; ++n; }
+    if (a > 0 && b > 0) { pis[n] = M - a; pjs[n] = Z - b; ++n; }
+    npen = n;
+  }
+  __syncthreads();
+  const int np = npen;
+  const int nv = 2 * np;
+  const double kxa = S.kx[a], kzb = S.kz[b];
+  const double k2 = kxa * kxa + kzb * kzb;
+  const size_t kofs = (size_t)g * N * N;
+
+  // ---- load v, omega_y = (i kz) u - (i kx) w, and the explicit terms
+  for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
+    const int p = it / N, l = it % N;
+    const int i = pis[p], j = pjs[p];
+    const size_t e = ((size_t)i * N + l) * Z + j;
+    const double kxv = S.kx[i], kzv = S.kz[j];
+    const double2 u = S.U[e], v = S.U[S.cs + e], w = S.U[2 * S.cs + e];
+    sv[(2 * p) * JSD_NYMAX + l] = v.x;
+    sv[(2 * p + 1) * JSD_NYMAX + l] = v.y;
+    const double2 om = csub(cik(kzv, u), cik(kxv, w));
+    sw[(2 * p) * JSD_NYMAX + l] = om.x;
+    sw[(2 * p + 1) * JSD_NYMAX + l] = om.y;
+    double2 nv_ = cscale(S.HV[e], S.dt_gamma);
+    double2 ng_ = cscale(S.HG[e], S.dt_gamma);
+    if (S.use_old) {
+      nv_ = cadd(nv_, cscale(S.HVo[e], S.dt_xi));
+      ng_ = cadd(ng_, cscale(S.HGo[e], S.dt_xi));
+    }
+    snl[(2 * p) * JSD_NYMAX + l] = nv_.x;
+    snl[(2 * p + 1) * JSD_NYMAX + l] = nv_.y;
+    snl[(nv + 2 * p) * JSD_NYMAX + l] = ng_.x;
+    snl[(nv + 2 * p + 1) * JSD_NYMAX + l] = ng_.y;
+  }
+  __syncthreads();
+  // ---- phi = ((-kx^2) v + D^2 v) + (-kz^2) v    (navier_stokes.py:1716-1725)
+  mv_lds(S.D2_T, N, sv, srhs, nv, 0, N);
+  __syncthreads();
+  for (int it = threadIdx.x; it < nv * N; it += blockDim.x) {
+    const int v = it / N, l = it % N;
+    const int p = v / 2;
+    const double kxv = S.kx[pis[p]], kzv = S.kz[pjs[p]];
+    const double x = sv[v * JSD_NYMAX + l];
+    sphi[v * JSD_NYMAX + l] = ((-(kxv * kxv)) * x + srhs[v * JSD_NYMAX + l]) + (-(kzv * kzv)) * x;
+    sphi[(nv + v) * JSD_NYMAX + l] = sw[v * JSD_NYMAX + l];
+  }
+  __syncthreads();
+  // ---- rhs = BC(R_s [phi | omega] + explicit)   (navier_stokes.py:1296-1302, :1400-1408)
+  mv_lds(S.RB_T, N, sphi, srhs, 2 * nv, 1, N - 1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < 2 * nv * N; it += blockDim.x) {
+    const int v = it / N, l = it % N;
+    double r = 0.0;
+    if (l > 0 && l < N - 1) r = (srhs[v * JSD_NYMAX + l] + S.diag_alpha * (-k2) * sphi[v * JSD_NYMAX + l]) + snl[v * JSD_NYMAX + l];
+    sphi[v * JSD_NYMAX + l] = r;
+  }
+  __syncthreads();
+  // ---- [phi' | omega'] = BC(L_s^-1 rhs)
+  mv_lds(S.LI_T + kofs, N, sphi, srhs, 2 * nv, 1, N - 1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < 2 * nv * 2; it += blockDim.x) {
+    const int v = it / 2;
+    srhs[v * JSD_NYMAX + (it % 2 ? N - 1 : 0)] = 0.0;
+  }
+  __syncthreads();
+  // ---- v_p = BC(P phi')
+  mv_lds(S.PO_T + (size_t)g * N * N, N, srhs, sphi, nv, 1, N - 1);
+  __syncthreads();
+  for (int it = threadIdx.x; it < nv * 2; it += blockDim.x) {
+    const int v = it / 2;
+    sphi[v * JSD_NYMAX + (it % 2 ? N - 1 : 0)] = 0.0;
+  }
+  __syncthreads();
+  // ---- wall correction: [a, b] = Minv (-[D v_p]_0, -[D v_p]_{N-1})
+  if (threadIdx.x < nv) {
+    const int v = threadIdx.x;
+    double d0 = 0.0, d1 = 0.0;
+    for (int j = 0; j < N; ++j) {
+      const double x = sphi[v * JSD_NYMAX + j];
+      d0 += S.D_T[(size_t)j * N + 0] * x;
+      d1 += S.D_T[(size_t)j * N + (N - 1)] * x;
+    }
+    const double* mi = S.INFL + (size_t)g * 4;
+    const double r0 = -d0, r1 = -d1;
+    sab[v * 2 + 0] = mi[0] * r0 + mi[1] * r1;
+    sab[v * 2 + 1] = mi[2] * r0 + mi[3] * r1;
+  }
+  __syncthreads();
+  const double* va = S.VA + (size_t)g * N;
+  for (int it = threadIdx.x; it < nv * N; it += blockDim.x) {
+    const int v = it / N, l = it % N;
+    double x = 0.0;
+    if (l > 0 && l < N - 1)
+      x = (sphi[v * JSD_NYMAX + l] + sab[v * 2 + 0] * va[l]) + sab[v * 2 + 1] * va[N - 1 - l];
+    sv[v * JSD_NYMAX + l] = x;  // new v (BC applied)
+    sw[v * JSD_NYMAX + l] = (l > 0 && l < N - 1) ? srhs[(nv + v) * JSD_NYMAX + l] : 0.0;  // new omega
+  }
+  __syncthreads();
+  // ---- D v_new
+  mv_lds(S.D_T, N, sv, srhs, nv, 0, N);
+  __syncthreads();
+  // ---- write u, v, w
+  for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
+    const int p = it / N, l = it % N;
+    const int i = pis[p], j = pjs[p];
+    const size_t e = ((size_t)i * N + l) * Z + j;
+    const bool wall = (l == 0 || l == N - 1);
+    const double2 vnew = make_double2(sv[(2 * p) * JSD_NYMAX + l], sv[(2 * p + 1) * JSD_NYMAX + l]);
+    S.U[S.cs + e] = vnew;
+    if (i == 0 && j == 0) continue;  // mean flow below
+    const double kxv = S.kx[i], kzv = S.kz[j];
+    const double mk2 = -(kxv * kxv + kzv * kzv);
+    const double dr = srhs[(2 * p) * JSD_NYMAX + l], di = srhs[(2 * p + 1) * JSD_NYMAX + l];
+    const double orr = sw[(2 * p) * JSD_NYMAX + l], oi = sw[(2 * p + 1) * JSD_NYMAX + l];
+    // u = (-(i kx) Dv + (i kz) om) / (-k^2); w = (-(i kz) Dv - (i kx) om) / (-k^2)
+    double2 u = make_double2((kxv * di + (-kzv * oi)) / mk2, (-kxv * dr + kzv * orr) / mk2);
+    double2 w = make_double2((kzv * di - (-kxv * oi)) / mk2, (-kzv * dr - kxv * orr) / mk2);
+    if (wall) {
+      u = make_double2(0.0, 0.0);
+      w = make_double2(0.0, 0.0);
+    }
+    S.U[e] = u;
+    S.U[2 * S.cs + e] = w;
+  }
+  if (a != 0 || b != 0) return;
+  __syncthreads();
+  // ---- (0,0) mean flow: [u;w] = Lns^-1 (Rns [u;w] + dt g N_new + dt x N_old)  (navier_stokes.py:1417-1504)
+  double* x2 = snl;   // 2N complex as [re 2N][im 2N]
+  double* y2 = sphi;
+  const int N2 = 2 * N;
+  for (int l = threadIdx.x; l < N2; l += blockDim.x) {
+    const size_t e = ((size_t)(l % N)) * Z;  // (i=0, l, j=0)
+    const double2 val = (l < N) ? S.U[e] : S.U[2 * S.cs + e];
+    x2[l] = val.x;
+    x2[N2 + l] = val.y;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < N2; r += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < N2; ++c) {
+      const double m = S.RNS[(size_t)r * N2 + c];
+      sr += m * x2[c];
+      si += m * x2[N2 + c];
+    }
+    const int comp = r < N ? 0 : 1;
+    const int l = r % N;
+    const double2 cn = S.CONV[comp * N + l];
+    double nr = -cn.x, ni = -cn.y;
+    if (comp == 0) nr += -S.dpdx;
+    double tr = sr + S.dt_gamma * nr, ti = si + S.dt_gamma * ni;
+    if (S.use_old) {
+      const double2 co = S.CONVo[comp * N + l];
+      double orr = -co.x, oi = -co.y;
+      if (comp == 0) orr += -S.dpdx;
+      tr += S.dt_xi * orr;
+      ti += S.dt_xi * oi;
+    }
+    y2[r] = tr;
+    y2[N2 + r] = ti;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < N2; r += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < N2; ++c) {
+      const double m = S.LNS[(size_t)r * N2 + c];
+      sr += m * y2[c];
+      si += m * y2[N2 + c];
+    }
+    const int l = r % N;
+    const bool wall = (l == 0 || l == N - 1);
+    const size_t e = (size_t)l * Z;
+    const double2 val = wall ? make_double2(0.0, 0.0) : make_double2(sr, si);
+    if (r < N) S.U[e] = val;
+    else S.U[2 * S.cs + e] = val;
+  }
+}
+
+// ===========================================================================
+// energy: E = sum_c sum_l (w_l / 4) s^2/(Px Pz) sum_k |Herm(u_c)|^2
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_energy_partial(const double2* __restrict__ U, long cs,
+                                                        const double* __restrict__ wy, int M, int N, int Z,
+                                                        double* __restrict__ partial) {
+  __shared__ double red[256];
+  const size_t tot = (size_t)3 * M * N * Z;
+  double acc = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = e / cs;
+    const size_t r = e - (size_t)c * cs;
+    const int j = r % Z;
+    const int l = (r / Z) % N;
+    const int i = r / ((size_t)Z * N);
+    const size_t p = ((size_t)((M - i) % M) * N + l) * Z + (Z - j) % Z;
+    const double2 h = cherm(U[e], U[(size_t)c * cs + p]);
+    acc += wy[l] * (h.x * h.x + h.y * h.y);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_energy_final(const double* __restrict__ partial, int n, double scale, double* __restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+static thread_local std::string g_err;
+
+#define JSD_FAIL(code, ...)                                 \
+  do {                                                      \
+    char _b[512];                                           \
+    snprintf(_b, sizeof(_b), __VA_ARGS__);                  \
+    g_err = _b;                                             \
+    return code;                                            \
+  } while (0)
+
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t _e = (x);                                                                           \
+    if (_e != hipSuccess) JSD_FAIL(JSD_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e), \
+                                   __FILE__, __LINE__);                                            \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct jsd_plan {
+  jsd_plan_desc d;
+  int M, N, Z, K, KZ, Ka, Kb, NT, KS;
+  size_t fs;   // complex elements per field
+  double s_inv, s_fwd, dpdx_scale;
+  std::vector<void*> allocs;
+  // device tables
+  double *DF1, *DF2, *D_T, *D2_T, *RB_T, *LI_T, *PO_T, *VA, *INFL, *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
+  int *permx, *permz;
+  double2 *twx, *twz;
+  FftPlan fpx, fpz;
+  // workspaces
+  double2 *DU, *H, *HV, *HG, *HVo, *HGo, *CONV, *CONVo, *T, *V, *WF, *DWF;
+  double* partial;
+  double* eout;
+  int npartial;
+};
+
+static int dalloc(jsd_plan* p, void** ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) JSD_FAIL(JSD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  p->allocs.push_back(*ptr);
+  return JSD_OK;
+}
+
+template <typename T>
+static int upload(jsd_plan* p, T** dst, const T* src, size_t count) {
+  int rc = dalloc(p, (void**)dst, count * sizeof(T));
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return JSD_OK;
+}
+
+static void transpose_sq(const double* src, double* dst, int n) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) dst[(size_t)j * n + i] = src[(size_t)i * n + j];
+}
+
+static int factor_plan(int p, std::vector<int>& rad) {
+  static const int cand[6] = {8, 7, 5, 4, 3, 2};
+  int rest = p;
+  rad.clear();
+  for (int c : cand)
+    while (rest % c == 0 && rest > 1) {
+      rad.push_back(c);
+      rest /= c;
+    }
+  return rest == 1 && (int)rad.size() <= JSD_MAXRAD;
+}
+
+// DIT input order (outermost radix = last entry), same recursion as the host plan
+static void dit_order(int len, const std::vector<int>& rs, int nrs, std::vector<int>& out) {
+  if (len == 1) {
+    out.assign(1, 0);
+    return;
+  }
+  const int r = rs[nrs - 1];
+  const int q = len / r;
+  std::vector<int> sub;
+  dit_order(q, rs, nrs - 1, sub);
+  out.clear();
+  for (int n2 = 0; n2 < r; ++n2)
+    for (int n1 : sub) out.push_back(r * n1 + n2);
+}
+
+static int make_fft(jsd_plan* p, int len, FftPlan* fp, int** perm, double2** tw) {
+  std::vector<int> rad;
+  if (!factor_plan(len, rad)) JSD_FAIL(JSD_ERR_ARG, "transform length %d is not a product of 2,3,4,5,7,8", len);
+  std::vector<int> order;
+  dit_order(len, rad, (int)rad.size(), order);
+  std::vector<double2> t(len);
+  for (int m = 0; m < len; ++m) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)len;
+    t[m] = make_double2((double)cosl(ang), -(double)sinl(ang));
+  }
+  int rc = upload(p, perm, order.data(), len);
+  if (rc) return rc;
+  rc = upload(p, tw, t.data(), len);
+  if (rc) return rc;
+  fp->p = len;
+  fp->nrad = (int)rad.size();
+  for (int s = 0; s < fp->nrad; ++s) fp->rad[s] = rad[s];
+  fp->perm = *perm;
+  fp->tw = *tw;
+  return JSD_OK;
+}
+
+// fragment-tiled D^order for the MFMA A operand
+static void tile_D(const double* Dm, int N, int NT, int KS, std::vector<double>& out) {
+  out.assign((size_t)NT * KS * 64, 0.0);
+  for (int lt = 0; lt < NT; ++lt)
+    for (int ks = 0; ks < KS; ++ks)
+      for (int lane = 0; lane < 64; ++lane) {
+        const int r = lt * 16 + (lane & 15), c = ks * 4 + (lane >> 4);
+        out[((size_t)lt * KS + ks) * 64 + lane] = (r < N && c < N) ? Dm[(size_t)r * N + c] : 0.0;
+      }
+}
+
+extern "C" int jsd_version(void) { return 1; }
+extern "C" const char* jsd_last_error(void) { return g_err.c_str(); }
+
+extern "C" int jsd_plan_destroy(jsd_plan* p) {
+  if (!p) return JSD_OK;
+  for (void* a : p->allocs) (void)hipFree(a);
+  delete p;
+  return JSD_OK;
+}
+
+extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
+  if (!d || !out) JSD_FAIL(JSD_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int M = d->nxm, N = d->ny, Z = d->nzm;
+  if (M < 1 || Z < 1 || (M % 2) == 0 || (Z % 2) == 0) JSD_FAIL(JSD_ERR_ARG, "mode counts must be odd (%d, %d)", M, Z);
+  if (N < 4 || N > JSD_NYMAX) JSD_FAIL(JSD_ERR_ARG, "ny=%d outside [4, %d]", N, JSD_NYMAX);
+  if (d->kind < 0 || d->kind > 2) JSD_FAIL(JSD_ERR_ARG, "unknown kind %d", d->kind);
+  if (d->ppx < 3 * (M / 2) + 1 || d->ppz < 3 * (Z / 2) + 1)
+    JSD_FAIL(JSD_ERR_ARG, "transform grid (%d, %d) cannot dealias (%d, %d) modes", d->ppx, d->ppz, M, Z);
+  if (!d->D || !d->kx || !d->kz || !d->lhs_inv || !d->poisson || !d->v_a || !d->infl_inv || !d->rhs_base ||
+      !d->rhs_ns || !d->lhs_inv_ns || !d->y_weights)
+    JSD_FAIL(JSD_ERR_ARG, "missing operator table");
+  if (d->kind != JSD_KIND_NS && (!d->base_u || !d->base_du)) JSD_FAIL(JSD_ERR_ARG, "missing base-flow profile");
+  jsd_plan* p = new jsd_plan();
+  p->d = *d;
+  p->M = M;
+  p->N = N;
+  p->Z = Z;
+  p->K = M / 2;
+  p->KZ = Z / 2;
+  p->Ka = p->K + 1;
+  p->Kb = p->KZ + 1;
+  p->NT = (N + 15) / 16;
+  p->KS = (N + 3) / 4;
+  p->fs = (size_t)M * N * Z;
+  const double sl = (d->lx / (2 * M_PI)) * (d->lz / (2 * M_PI));
+  p->s_inv = sl / sqrt((double)d->px * (double)d->pz);
+  p->s_fwd = sqrt((double)d->px * (double)d->pz) / (sl * (double)d->ppx * (double)d->ppz);
+  p->dpdx_scale = pow((d->px * pow(2 * M_PI / d->lx, 2)) * (d->pz * pow(2 * M_PI / d->lz, 2)), 0.5);
+  int rc = JSD_OK;
+#define TRY(x)            \
+  do {                    \
+    rc = (x);             \
+    if (rc) {             \
+      jsd_plan_destroy(p); \
+      return rc;          \
+    }                     \
+  } while (0)
+  const size_t nn = (size_t)N * N;
+  const size_t nk = (size_t)p->Ka * p->Kb;
+  // D, D^2 (numpy matrix_power order: D @ D)
+  std::vector<double> D2(nn, 0.0), tmp(nn);
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < N; ++k) {
+      const double a = d->D[(size_t)i * N + k];
+      for (int j = 0; j < N; ++j) D2[(size_t)i * N + j] += a * d->D[(size_t)k * N + j];
+    }
+  std::vector<double> tiled;
+  tile_D(d->D, N, p->NT, p->KS, tiled);
+  TRY(upload(p, &p->DF1, tiled.data(), tiled.size()));
+  tile_D(D2.data(), N, p->NT, p->KS, tiled);
+  TRY(upload(p, &p->DF2, tiled.data(), tiled.size()));
+  transpose_sq(d->D, tmp.data(), N);
+  TRY(upload(p, &p->D_T, tmp.data(), nn));
+  transpose_sq(D2.data(), tmp.data(), N);
+  TRY(upload(p, &p->D2_T, tmp.data(), nn));
+  {
+    std::vector<double> rb(3 * nn), li(3 * nk * nn), po(nk * nn);
+    for (int s = 0; s < 3; ++s) transpose_sq(d->rhs_base + s * nn, rb.data() + s * nn, N);
+    for (size_t k = 0; k < 3 * nk; ++k) transpose_sq(d->lhs_inv + k * nn, li.data() + k * nn, N);
+    for (size_t k = 0; k < nk; ++k) transpose_sq(d->poisson + k * nn, po.data() + k * nn, N);
+    TRY(upload(p, &p->RB_T, rb.data(), rb.size()));
+    TRY(upload(p, &p->LI_T, li.data(), li.size()));
+    TRY(upload(p, &p->PO_T, po.data(), po.size()));
+  }
+  TRY(upload(p, &p->VA, d->v_a, 3 * nk * N));
+  TRY(upload(p, &p->INFL, d->infl_inv, 3 * nk * 4));
+  TRY(upload(p, &p->RNS, d->rhs_ns, 3 * 4 * nn));
+  TRY(upload(p, &p->LNS, d->lhs_inv_ns, 3 * 4 * nn));
+  TRY(upload(p, &p->kx, d->kx, M));
+  TRY(upload(p, &p->kz, d->kz, Z));
+  TRY(upload(p, &p->wy, d->y_weights, N));
+  if (d->kind != JSD_KIND_NS) {
+    TRY(upload(p, &p->Ub, d->base_u, N));
+    TRY(upload(p, &p->dUb, d->base_du, N));
+  } else {
+    p->Ub = p->dUb = nullptr;
+  }
+  TRY(make_fft(p, d->ppx, &p->fpx, &p->permx, &p->twx));
+  TRY(make_fft(p, d->ppz, &p->fpz, &p->permz, &p->twz));
+  const size_t fb = p->fs * sizeof(double2);
+  TRY(dalloc(p, (void**)&p->DU, 3 * fb));
+  TRY(dalloc(p, (void**)&p->H, 3 * fb));
+  TRY(dalloc(p, (void**)&p->HV, fb));
+  TRY(dalloc(p, (void**)&p->HG, fb));
+  TRY(dalloc(p, (void**)&p->HVo, fb));
+  TRY(dalloc(p, (void**)&p->HGo, fb));
+  TRY(dalloc(p, (void**)&p->CONV, 2 * N * sizeof(double2)));
+  TRY(dalloc(p, (void**)&p->CONVo, 2 * N * sizeof(double2)));
+  const bool nonlin = (d->kind == JSD_KIND_NS) || !d->linearise;
+  if (nonlin) {
+    const int ntr = (d->kind == JSD_KIND_DUAL) ? 12 : 6;
+    const size_t ts = (size_t)d->ppx * N * Z;
+    TRY(dalloc(p, (void**)&p->T, ntr * ts * sizeof(double2)));
+    TRY(dalloc(p, (void**)&p->V, 2 * ts * sizeof(double2)));
+  } else {
+    p->T = p->V = nullptr;
+  }
+  if (d->kind == JSD_KIND_DUAL && !d->linearise) {
+    TRY(dalloc(p, (void**)&p->DWF, 3 * fb));
+  } else {
+    p->DWF = nullptr;
+  }
+  p->WF = nullptr;
+  p->npartial = 1024;
+  TRY(dalloc(p, (void**)&p->partial, p->npartial * sizeof(double)));
+  TRY(dalloc(p, (void**)&p->eout, sizeof(double)));
+#undef TRY
+  *out = p;
+  return JSD_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int ncomp, int order, hipStream_t st) {
+  dim3 grid(p->M, (2 * p->Z + 63) / 64, ncomp);
+  hipLaunchKernelGGL(k_ygemm<GEMM_PLAIN>, grid, dim3(256), 0, st, order == 2 ? p->DF2 : p->DF1, p->NT, p->KS,
+                     (const double*)in, nullptr, nullptr, (double*)out, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
+                     p->Z, (long)(2 * p->fs));
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+static int launch_hv(jsd_plan* p, const double2* H, double2* hv, double2* hg, double2* conv, hipStream_t st) {
+  dim3 grid(p->M, (2 * p->Z + 63) / 64, 1);
+  const double* h = (const double*)H;
+  hipLaunchKernelGGL(k_ygemm<GEMM_HV>, grid, dim3(256), 0, st, p->DF1, p->NT, p->KS, h, h + 2 * p->fs,
+                     h + 4 * p->fs, (double*)hv, (double*)hg, (double*)conv, p->kx, p->kz, p->M, p->N, p->Z, 0L);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+// H (3 comps, stored convention) from the state U (and the forward state for a nonlinear dual)
+static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStream_t st) {
+  const jsd_plan_desc& d = p->d;
+  const size_t fs = p->fs;
+  const int M = p->M, N = p->N, Z = p->Z;
+  const bool lin = (d.kind != JSD_KIND_NS) && d.linearise;
+  if (lin) {
+    int rc = launch_ygemm_plain(p, U + fs, p->DU, 1, 1, st);
+    if (rc) return rc;
+    const size_t tot = fs;
+    hipLaunchKernelGGL(k_nl_lin, dim3((tot + 255) / 256), dim3(256), 0, st, U, (long)fs, p->DU, p->H, p->kx,
+                       p->kz, p->Ub, p->dUb, M, N, Z, d.kind == JSD_KIND_DUAL ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    return JSD_OK;
+  }
+  int rc = launch_ygemm_plain(p, U, p->DU, 3, 1, st);
+  if (rc) return rc;
+  const int nqb = (p->KZ + 1 + QB - 1) / QB;
+  const size_t ts = (size_t)d.ppx * N * Z;
+  const size_t lds_x = (size_t)d.ppx * NSLOT * sizeof(double2);
+  int ntr = 6;
+  for (int c = 0; c < 3; ++c) {
+    hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, U + c * fs, (int)OP_ID, U + c * fs,
+                       (int)OP_DX, p->T + (2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
+    hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, p->DU + c * fs, (int)OP_ID, U + c * fs,
+                       (int)OP_DZ, p->T + (2 * c + 1) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
+  }
+  int prod = PROD_NS;
+  double clin = 1.0;
+  if (d.kind == JSD_KIND_PERTURBATION) prod = PROD_PERT;
+  if (d.kind == JSD_KIND_DUAL) {
+    prod = PROD_DUAL;
+    ntr = 12;
+    if (!UF) JSD_FAIL(JSD_ERR_ARG, "nonlinear dual step needs the forward state");
+    rc = launch_ygemm_plain(p, UF, p->DWF, 3, 1, st);
+    if (rc) return rc;
+    for (int c = 0; c < 3; ++c) {
+      hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, UF + c * fs, (int)OP_ID, UF + c * fs,
+                         (int)OP_DX, p->T + (6 + 2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
+      hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, p->DWF + c * fs, (int)OP_ID,
+                         UF + c * fs, (int)OP_DZ, p->T + (7 + 2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z,
+                         p->s_inv);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  const size_t lds_z = ((size_t)d.ppz * ntr + (size_t)d.ppz * 4) * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, N), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V, (long)ts, p->fpz,
+                     N, Z, prod, clin, p->Ub, p->dUb);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fft_x_fwd, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx, M,
+                     N, Z, p->s_fwd);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st) {
+  static const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  static const double xi[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  static const double alp[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
+  int rc = nonlinear_H(p, U, UF, st);
+  if (rc) return rc;
+  rc = launch_hv(p, p->H, p->HV, p->HG, p->CONV, st);
+  if (rc) return rc;
+  const size_t nn = (size_t)p->N * p->N;
+  const size_t nk = (size_t)p->Ka * p->Kb;
+  SolveArgs S;
+  S.U = U;
+  S.cs = (long)p->fs;
+  S.HV = p->HV;
+  S.HG = p->HG;
+  S.HVo = p->HVo;
+  S.HGo = p->HGo;
+  S.CONV = p->CONV;
+  S.CONVo = p->CONVo;
+  S.D_T = p->D_T;
+  S.D2_T = p->D2_T;
+  S.RB_T = p->RB_T + s * nn;
+  S.LI_T = p->LI_T + (size_t)s * nk * nn;
+  S.PO_T = p->PO_T;
+  S.VA = p->VA + (size_t)s * nk * p->N;
+  S.INFL = p->INFL + (size_t)s * nk * 4;
+  S.RNS = p->RNS + (size_t)s * 4 * nn;
+  S.LNS = p->LNS + (size_t)s * 4 * nn;
+  S.kx = p->kx;
+  S.kz = p->kz;
+  S.M = p->M;
+  S.N = p->N;
+  S.Z = p->Z;
+  S.Ka = p->Ka;
+  S.Kb = p->Kb;
+  const double dt = p->d.dt;
+  S.dt_gamma = dt * gam[s];
+  S.dt_xi = dt * xi[s];
+  // alpha_s dt (-k^2)/Re enters as diag_alpha * (-k^2) with diag_alpha = alpha_s*dt/Re;
+  // the tables were built with dt_tab/Re_tab = dt/Re (the dual flips both signs)
+  S.diag_alpha = alp[s] * fabs(dt) / fabs(p->d.re_tau);
+  S.dpdx = dPdx * p->dpdx_scale;
+  S.use_old = s > 0;
+  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, S);
+  HIPCHK(hipGetLastError());
+  std::swap(p->HV, p->HVo);
+  std::swap(p->HG, p->HGo);
+  std::swap(p->CONV, p->CONVo);
+  return JSD_OK;
+}
+
+extern "C" int jsd_step(jsd_plan* p, void* vel_hat, double dPdx, int nsteps, void* stream) {
+  if (!p || !vel_hat || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (p->d.kind == JSD_KIND_DUAL && !p->d.linearise)
+    JSD_FAIL(JSD_ERR_STATE, "a nonlinear dual plan steps with jsd_dual_step");
+  hipStream_t st = (hipStream_t)stream;
+  for (int n = 0; n < nsteps; ++n)
+    for (int s = 0; s < 3; ++s) {
+      int rc = substep(p, (double2*)vel_hat, nullptr, s, dPdx, st);
+      if (rc) return rc;
+    }
+  return JSD_OK;
+}
+
+extern "C" int jsd_dual_step(jsd_plan* p, void* adj_hat, const void* fwd_hat, void* stream) {
+  if (!p || !adj_hat) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (p->d.kind != JSD_KIND_DUAL) JSD_FAIL(JSD_ERR_STATE, "not a dual plan");
+  hipStream_t st = (hipStream_t)stream;
+  for (int s = 0; s < 3; ++s) {
+    int rc = substep(p, (double2*)adj_hat, (const double2*)fwd_hat, s, 0.0, st);
+    if (rc) return rc;
+  }
+  return JSD_OK;
+}
+
+__global__ void k_negate(const double* __restrict__ x, double* __restrict__ y, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = -x[i];
+}
+
+extern "C" int jsd_nonlinear(jsd_plan* p, const void* vel_hat, const void* fwd_hat, void* h_v, void* h_g,
+                             void* conv_ns, void* stream) {
+  if (!p || !vel_hat || !h_v || !h_g || !conv_ns) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = nonlinear_H(p, (const double2*)vel_hat, (const double2*)fwd_hat, st);
+  if (rc) return rc;
+  rc = launch_hv(p, p->H, (double2*)h_v, (double2*)h_g, p->CONV, st);
+  if (rc) return rc;
+  // conv_ns = -H
+  const size_t n = 6 * p->fs;
+  hipLaunchKernelGGL(k_negate, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)p->H, (double*)conv_ns, n);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+extern "C" int jsd_energy(jsd_plan* p, const void* vel_hat, double* out, void* stream) {
+  if (!p || !vel_hat || !out) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  hipStream_t st = (hipStream_t)stream;
+  const double sl = (p->d.lx / (2 * M_PI)) * (p->d.lz / (2 * M_PI));
+  const double scale = 0.25 * sl * sl / ((double)p->d.px * (double)p->d.pz);
+  hipLaunchKernelGGL(k_energy_partial, dim3(p->npartial), dim3(256), 0, st, (const double2*)vel_hat, (long)p->fs,
+                     p->wy, p->M, p->N, p->Z, p->partial);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, p->partial, p->npartial, scale, p->eout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, p->eout, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return JSD_OK;
+}
+
+extern "C" int jsd_ydiff(jsd_plan* p, const void* in, void* out, int ncomp, int order, void* stream) {
+  if (!p || !in || !out || ncomp < 1 || (order != 1 && order != 2)) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  return launch_ygemm_plain(p, (const double2*)in, (double2*)out, ncomp, order, (hipStream_t)stream);
+}

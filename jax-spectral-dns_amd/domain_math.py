@@ -1,0 +1,271 @@
+"""Host-side grid and operator precompute for the channel-flow step.
+
+Everything here runs once per (shape, Re, dt) on the host in NumPy FP64 and
+is handed to the HIP library as plain arrays.  The formulas are those of the
+reference's Domain and operator assembly:
+
+* Gauss-Lobatto grid and collocation derivative matrix D
+  (reference domain.py:130-163),
+* the odd-mode / physical-size rule (domain.py:217-258, 143-152),
+* stored wavenumbers [0..K, -K..-1] 2 pi / L (domain.py:796-827),
+* RK3/CN operator tables: R_s = hom_dir(I + alpha_s dt L), the inverse of
+  hom_dir(I - beta_s dt L), the Poisson inverse
+  inv(-(kx^2+kz^2) eye_bc + hom_dir(D^2)), the mean-flow 2Ny block operators
+  (navier_stokes.py:920-1022, domain.py:857-889),
+* the y quadrature of the definite integral (domain.py:577-628).
+
+The tables depend on kx^2 + kz^2 only, so they are stored once per distinct
+(|kx|, |kz|) pair, in real FP64 (the reference stores the same values per
+signed pair as complex128).  Two derived per-(|k|, substep) quantities are
+precomputed as well: the homogeneous-Neumann correction solution
+v_a = BC(P (L_inh^-1 e_0)) and the inverse of the 2x2 influence matrix built
+from D v_a and D flip(v_a) at the walls (navier_stokes.py:1321-1389).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+RK_ALPHA = (29 / 96, -3 / 40, 1 / 6)
+RK_BETA = (37 / 160, 5 / 24, 1 / 6)
+RK_GAMMA = (8 / 15, 5 / 12, 3 / 4)
+RK_XI = (0.0, -17 / 60, -5 / 12)
+
+# radices the HIP FFT kernel implements
+FFT_RADICES = (8, 7, 5, 4, 3, 2)
+
+
+def modes_and_physical(n: int, aliasing: float = 1.5) -> Tuple[int, int]:
+    """Stored odd mode count and the reference's physical grid size for a
+    periodic direction (domain.py:217-258 with get_fourier_grid dropping the
+    last point of an odd argument, domain.py:143-152)."""
+    m = int(n) if int(n) % 2 != 0 else int(n) + 1
+    c = math.ceil(m * aliasing)
+    arg = c if c % 2 != 0 else c + 1
+    return m, (arg - 1 if arg % 2 != 0 else arg)
+
+
+def factorize(n: int) -> List[int]:
+    """Radix list (innermost first) using the kernel's radices, or [] if n has
+    another prime factor."""
+    out: List[int] = []
+    rest = n
+    for r in FFT_RADICES:
+        while rest % r == 0 and rest > 1:
+            out.append(r)
+            rest //= r
+    return out if rest == 1 else []
+
+
+def transform_size(m: int, reference_p: int) -> int:
+    """Smallest P' >= 3K+1 (K = m // 2) made of the kernel's radices.  Any such
+    grid computes the same |k| <= K spectrum of a product of two fields with
+    |k| <= K as the reference's grid, up to rounding."""
+    k = m // 2
+    p = max(3 * k + 1, 2)
+    while not factorize(p):
+        p += 1
+    return p
+
+
+def cheb_grid(n: int) -> np.ndarray:
+    return np.array([np.cos(np.pi / (n - 1) * i) for i in np.arange(n)])
+
+
+def cheb_diff(xs: np.ndarray) -> np.ndarray:
+    n = len(xs)
+    c = np.block([2.0, np.ones((1, n - 2)) * 1.0, 2.0]) * (-1) ** np.arange(0, n)
+    xm = np.repeat(xs, n).reshape(n, n)
+    dx = xm - xm.T
+    d_ = ((1 / c).T @ c).T / (dx + np.eye(n))
+    return d_ - np.diag(sum(d_.T))
+
+
+def wavenumbers(m: int, length: float) -> np.ndarray:
+    inp = np.linspace(0, m - 1, int(m))
+    return (np.block([inp[m // 2 :], inp[: m // 2]]) - m // 2) * (2 * np.pi) / length
+
+
+def hom_dir(mat: np.ndarray) -> np.ndarray:
+    out = mat.copy()
+    out[0, :] = 0.0
+    out[:, 0] = 0.0
+    out[-1, :] = 0.0
+    out[:, -1] = 0.0
+    out[0, 0] = 1.0
+    out[-1, -1] = 1.0
+    return out
+
+
+@dataclass
+class Grid:
+    nxm: int
+    ny: int
+    nzm: int
+    px: int  # reference physical sizes (they fix the spectral scaling)
+    pz: int
+    lx: float
+    lz: float
+    ppx: int  # transform grid used by the kernels
+    ppz: int
+    y: np.ndarray
+    D: np.ndarray
+    kx: np.ndarray
+    kz: np.ndarray
+    scaling: float
+    y_weights: np.ndarray
+    cache: Dict[str, object] = field(default_factory=dict)
+
+    @property
+    def D2(self) -> np.ndarray:
+        return np.linalg.matrix_power(self.D, 2)
+
+    @property
+    def ka(self) -> int:
+        return self.nxm // 2 + 1
+
+    @property
+    def kb(self) -> int:
+        return self.nzm // 2 + 1
+
+
+def make_grid(
+    shape: Tuple[int, int, int],
+    lx: float = 2 * np.pi,
+    lz: float = 2 * np.pi,
+    transform_grid: Tuple[int, int] = None,  # type: ignore[assignment]
+) -> Grid:
+    nxm, px = modes_and_physical(shape[0])
+    nzm, pz = modes_and_physical(shape[2])
+    ny = int(shape[1])
+    if nxm % 4 != 1 or nzm % 4 != 1:
+        # the reference pads ceil(m/2) zeros but its grid has one point less
+        # when m = 3 (mod 4); that case is inconsistent upstream
+        raise ValueError("stored mode counts must be 1 mod 4 (got %d, %d)" % (nxm, nzm))
+    if transform_grid is None:
+        ppx, ppz = transform_size(nxm, px), transform_size(nzm, pz)
+    else:
+        ppx, ppz = transform_grid
+    y = cheb_grid(ny)
+    D = cheb_diff(y)
+    scaling = 1.0
+    for length in (lx, lz):
+        scaling *= length / (2 * np.pi)
+    # y quadrature of the definite integral (integrate with bc_right = 0)
+    mat = D.copy()
+    mat[0, :] = 0.0
+    mat[:, 0] = 0.0
+    mat[0, 0] = 1.0
+    inv = np.linalg.inv(mat)
+    w = inv[0, :] - inv[ny - 1, :]
+    w[0] = 0.0
+    return Grid(nxm, ny, nzm, px, pz, lx, lz, ppx, ppz, y, D, wavenumbers(nxm, lx), wavenumbers(nzm, lz), scaling, w)
+
+
+@dataclass
+class OperatorTables:
+    lhs_inv: np.ndarray  # (3, Ka, Kb, Ny, Ny) row-major
+    poisson: np.ndarray  # (Ka, Kb, Ny, Ny)
+    v_a: np.ndarray  # (3, Ka, Kb, Ny)
+    infl_inv: np.ndarray  # (3, Ka, Kb, 4)  inverse 2x2 influence matrix, row-major
+    rhs_base: np.ndarray  # (3, Ny, Ny) hom_dir(I + alpha_s dt D2 / Re)
+    rhs_ns: np.ndarray  # (3, 2Ny, 2Ny)
+    lhs_inv_ns: np.ndarray  # (3, 2Ny, 2Ny)
+
+
+def build_operators(g: Grid, re_tau: float, dt: float) -> OperatorTables:
+    n = g.ny
+    ka, kb = g.ka, g.kb
+    D2 = np.linalg.matrix_power(g.D, 2)
+    Ly = 1 / re_tau * D2
+    I = np.eye(n)
+    kxa = g.kx[:ka]
+    kzb = g.kz[:kb]
+    k2 = (kxa[:, None] ** 2 + kzb[None, :] ** 2).reshape(-1)
+    nk = ka * kb
+    # L(k) = Ly + I * (-(kx^2+kz^2)) / Re, stacked over |k|
+    L = Ly[None, :, :] + I[None, :, :] * (-(k2))[:, None, None] / re_tau
+    lhs_inv = np.empty((3, nk, n, n))
+    v_a = np.empty((3, nk, n))
+    infl = np.empty((3, nk, 4))
+    rhs_base = np.empty((3, n, n))
+    eye_bc = np.eye(n)
+    eye_bc[0, 0] = 0.0
+    eye_bc[-1, -1] = 0.0
+    D2h = hom_dir(D2)
+    poisson = np.linalg.inv((-(k2))[:, None, None] * eye_bc[None] + D2h[None])
+    for s in range(3):
+        A = I[None] - RK_BETA[s] * dt * L
+        lhs = A.copy()
+        lhs[:, 0, :] = 0.0
+        lhs[:, :, 0] = 0.0
+        lhs[:, -1, :] = 0.0
+        lhs[:, :, -1] = 0.0
+        lhs[:, 0, 0] = 1.0
+        lhs[:, -1, -1] = 1.0
+        lhs_inv[s] = np.linalg.inv(lhs)
+        inh = A.copy()
+        inh[:, 0, :] = 0.0
+        inh[:, -1, :] = 0.0
+        inh[:, 0, 0] = 1.0
+        inh[:, -1, -1] = 1.0
+        phi_a = np.linalg.inv(inh)[:, :, 0]  # inv(L_inh) @ [1, 0, ..., 0]
+        va = np.einsum("kij,kj->ki", poisson, phi_a)
+        va[:, 0] = 0.0
+        va[:, -1] = 0.0
+        vb = va[:, ::-1]
+        dva = va @ g.D.T
+        dvb = vb @ g.D.T
+        M = np.stack([np.stack([dva[:, 0], dvb[:, 0]], -1), np.stack([dva[:, -1], dvb[:, -1]], -1)], -2)
+        infl[s] = np.linalg.inv(M).reshape(nk, 4)
+        v_a[s] = va
+        rhs_base[s] = hom_dir(I + RK_ALPHA[s] * dt * Ly)
+    Z = np.zeros((n, n))
+    L_ns = 1 / re_tau * np.block([[D2h, Z], [Z, D2h]])
+    I2 = np.eye(2 * n)
+    rhs_ns = np.stack([hom_dir(I2 + RK_ALPHA[s] * dt * L_ns) for s in range(3)])
+    lhs_inv_ns = np.stack([np.linalg.inv(hom_dir(I2 - RK_BETA[s] * dt * L_ns)) for s in range(3)])
+    return OperatorTables(
+        lhs_inv=lhs_inv.reshape(3, ka, kb, n, n),
+        poisson=poisson.reshape(ka, kb, n, n),
+        v_a=v_a.reshape(3, ka, kb, n),
+        infl_inv=infl.reshape(3, ka, kb, 4),
+        rhs_base=rhs_base,
+        rhs_ns=rhs_ns,
+        lhs_inv_ns=lhs_inv_ns,
+    )
+
+
+def dit_tables(p: int, inverse: bool) -> Tuple[np.ndarray, np.ndarray, List[int]]:
+    """In-place mixed-radix decimation-in-time FFT plan for length p:
+    (input index for each LDS position, twiddle table w_p^m, radix list with
+    the innermost radix first)."""
+    radices = factorize(p)
+    if not radices:
+        raise ValueError("unsupported FFT length %d" % p)
+
+    # recursive split, outermost radix = last entry: n = r * n1 + n2,
+    # sub-FFT n2 occupies positions [n2 * Q, (n2 + 1) * Q)
+    def order(length: int, rs: List[int]) -> List[int]:
+        if length == 1:
+            return [0]
+        r = rs[-1]
+        q = length // r
+        sub = order(q, rs[:-1])
+        out = []
+        for n2 in range(r):
+            out.extend(r * n1 + n2 for n1 in sub)
+        return out
+
+    perm = np.array(order(p, radices), dtype=np.int32)
+    sign = 1.0 if inverse else -1.0
+    # extended precision, then rounded: symmetric, correctly rounded twiddles
+    pi_l = np.longdouble("3.14159265358979323846264338327950288")
+    m = np.arange(p)
+    ang = 2 * pi_l * m.astype(np.longdouble) / p
+    tw = (np.cos(ang).astype(np.float64) + sign * 1j * np.sin(ang).astype(np.float64))
+    return perm, tw, radices

@@ -1,0 +1,164 @@
+"""Device plan: operator tables on the GPU plus the step's workspaces.
+
+``StepPlan`` owns one ``jsd_plan`` of libjsd.so.  State tensors are torch
+complex128 CUDA tensors of shape (3, Nx_modes, Ny, Nz_modes) (the reference's
+layout, navier_stokes.py:1778-1813); they are passed to the library as raw
+device pointers together with torch's current HIP stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .domain_math import Grid, OperatorTables, build_operators, make_grid
+
+
+def _ptr(a: np.ndarray) -> "ctypes._Pointer":
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dev(t: Optional[torch.Tensor]) -> Optional[ctypes.c_void_p]:
+    if t is None:
+        return None
+    if not t.is_cuda or t.dtype != torch.complex128 or not t.is_contiguous():
+        raise _lib.JsdError("expected a contiguous complex128 CUDA tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class StepPlan:
+    """A plan for one (grid, Re, dt, kind) combination."""
+
+    def __init__(
+        self,
+        grid: Grid,
+        re_tau: float,
+        dt: float,
+        kind: int,
+        linearise: bool = False,
+        u_max: float = 1.0,
+        ops: Optional[OperatorTables] = None,
+        explicit_dt: Optional[float] = None,
+        plan_re: Optional[float] = None,
+    ):
+        if not torch.cuda.is_available():
+            raise _lib.JsdError("no GPU visible: the channel-flow step runs only through the HIP library")
+        lib = _lib.load()
+        self.grid = grid
+        self.kind = kind
+        self.linearise = bool(linearise)
+        self.re_tau = re_tau
+        self.dt = dt
+        self.ops = ops if ops is not None else build_operators(grid, re_tau, dt)
+        g, o = grid, self.ops
+        self._keep = []
+
+        def arr(x: np.ndarray) -> np.ndarray:
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            self._keep.append(a)
+            return a
+
+        base_u = arr(u_max * (1 - g.y**2))
+        base_du = arr(g.D @ base_u)
+        desc = _lib.PlanDesc()
+        desc.nxm, desc.ny, desc.nzm = g.nxm, g.ny, g.nzm
+        desc.px, desc.pz, desc.ppx, desc.ppz = g.px, g.pz, g.ppx, g.ppz
+        desc.lx, desc.lz = g.lx, g.lz
+        desc.re_tau = plan_re if plan_re is not None else re_tau
+        desc.dt = explicit_dt if explicit_dt is not None else dt
+        desc.kind = kind
+        desc.linearise = int(self.linearise)
+        desc.D = _ptr(arr(g.D))
+        desc.kx = _ptr(arr(g.kx))
+        desc.kz = _ptr(arr(g.kz))
+        desc.lhs_inv = _ptr(arr(o.lhs_inv))
+        desc.poisson = _ptr(arr(o.poisson))
+        desc.v_a = _ptr(arr(o.v_a))
+        desc.infl_inv = _ptr(arr(o.infl_inv))
+        desc.rhs_base = _ptr(arr(o.rhs_base))
+        desc.rhs_ns = _ptr(arr(o.rhs_ns))
+        desc.lhs_inv_ns = _ptr(arr(o.lhs_inv_ns))
+        desc.base_u = _ptr(base_u)
+        desc.base_du = _ptr(base_du)
+        desc.y_weights = _ptr(arr(g.y_weights))
+        h = ctypes.c_void_p()
+        _lib.check(lib.jsd_plan_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+        self._keep = []  # tables were copied to the device
+
+    @property
+    def shape(self) -> Tuple[int, int, int, int]:
+        return (3, self.grid.nxm, self.grid.ny, self.grid.nzm)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().jsd_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_state(self, t: torch.Tensor) -> None:
+        if tuple(t.shape) != self.shape:
+            raise _lib.JsdError("state shape %s != %s" % (tuple(t.shape), self.shape))
+
+    def step(self, vel_hat: torch.Tensor, dPdx: float = 0.0, nsteps: int = 1) -> torch.Tensor:
+        """Advance vel_hat in place by nsteps steps."""
+        self._check_state(vel_hat)
+        _lib.check(_lib.load().jsd_step(self._h, _dev(vel_hat), float(dPdx), int(nsteps), _stream()))
+        return vel_hat
+
+    def dual_step(self, adj_hat: torch.Tensor, fwd_hat: Optional[torch.Tensor]) -> torch.Tensor:
+        self._check_state(adj_hat)
+        if fwd_hat is not None:
+            self._check_state(fwd_hat)
+        _lib.check(_lib.load().jsd_dual_step(self._h, _dev(adj_hat), _dev(fwd_hat), _stream()))
+        return adj_hat
+
+    def nonlinear(self, vel_hat: torch.Tensor, fwd_hat: Optional[torch.Tensor] = None):
+        self._check_state(vel_hat)
+        g = self.grid
+        h_v = torch.empty((g.nxm, g.ny, g.nzm), dtype=torch.complex128, device=vel_hat.device)
+        h_g = torch.empty_like(h_v)
+        conv = torch.empty_like(vel_hat)
+        _lib.check(
+            _lib.load().jsd_nonlinear(self._h, _dev(vel_hat), _dev(fwd_hat), _dev(h_v), _dev(h_g), _dev(conv), _stream())
+        )
+        return h_v, h_g, conv
+
+    def energy(self, vel_hat: torch.Tensor) -> float:
+        self._check_state(vel_hat)
+        out = ctypes.c_double(0.0)
+        _lib.check(_lib.load().jsd_energy(self._h, _dev(vel_hat), ctypes.byref(out), _stream()))
+        return out.value
+
+    def ydiff(self, f: torch.Tensor, order: int = 1) -> torch.Tensor:
+        ncomp = f.shape[0] if f.dim() == 4 else 1
+        out = torch.empty_like(f)
+        _lib.check(_lib.load().jsd_ydiff(self._h, _dev(f.contiguous()), _dev(out), ncomp, order, _stream()))
+        return out
+
+
+def make_plan(
+    shape: Tuple[int, int, int],
+    re: float,
+    dt: float,
+    kind: int,
+    lx: float = 2 * np.pi,
+    lz: float = 2 * np.pi,
+    linearise: bool = False,
+    u_max_over_u_tau: float = 1.0,
+) -> StepPlan:
+    grid = make_grid(shape, lx, lz)
+    return StepPlan(grid, re / u_max_over_u_tau, dt, kind, linearise, u_max_over_u_tau)

@@ -1,0 +1,114 @@
+"""Parity of the HIP path (through the C ABI) with the oracle, stage by stage
+and step by step, on seeded inputs.  Tolerances: <= 1e-11 relative L2 per
+step (BASELINE), nonlinear terms and derivatives <= 1e-12."""
+
+import numpy as np
+import pytest
+import torch
+
+from jax_spectral_dns_amd import KIND_DUAL, KIND_NS, KIND_PERTURBATION
+from jax_spectral_dns_amd.plan import StepPlan
+from oracle import jsd_oracle as O
+from tests.helpers import ns_field, pair, perturbation_field, rel
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+SHAPES = [((24, 65, 4), 2 * np.pi, 2 * np.pi), ((24, 33, 16), 2 * np.pi, np.pi)]
+
+
+@pytest.mark.parametrize("shape,lx,lz", SHAPES)
+def test_ydiff(shape, lx, lz):
+    g, d = pair(shape, lx, lz)
+    plan = StepPlan(g, 3000.0, 0.01, KIND_NS)
+    u = perturbation_field(g, seed=1, energy=1.0)
+    for order in (1, 2):
+        out = plan.ydiff(u.to(DEV), order).cpu()
+        ref = torch.stack([O.diff(d, u[c], 1, order) for c in range(3)])
+        assert rel(out, ref) < 1e-13, order
+
+
+@pytest.mark.parametrize("shape,lx,lz", SHAPES)
+def test_energy(shape, lx, lz):
+    g, d = pair(shape, lx, lz)
+    plan = StepPlan(g, 3000.0, 0.01, KIND_NS)
+    u = ns_field(g, seed=2)
+    e = plan.energy(u.to(DEV))
+    assert abs(e - float(O.energy(d, u))) / float(O.energy(d, u)) < 1e-12
+
+
+@pytest.mark.parametrize("shape,lx,lz", SHAPES)
+def test_nonlinear_ns(shape, lx, lz):
+    g, d = pair(shape, lx, lz)
+    plan = StepPlan(g, 5000.0, 0.005, KIND_NS)
+    u = ns_field(g, seed=3)
+    hv, hg, conv = plan.nonlinear(u.to(DEV))
+    rv, rg, _, rc = O.nl_ns_skew(d, u)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
+@pytest.mark.parametrize("linearise", [True, False])
+@pytest.mark.parametrize("shape,lx,lz", SHAPES)
+def test_nonlinear_perturbation(shape, lx, lz, linearise):
+    g, d = pair(shape, lx, lz)
+    plan = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=linearise)
+    u = perturbation_field(g, seed=4, energy=1e-2)
+    hv, hg, conv = plan.nonlinear(u.to(DEV))
+    base = O.base_flow_hat(d)
+    rv, rg, _, rc = O.nl_perturbation_skew(d, u, base, linearise)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
+@pytest.mark.parametrize("linearise", [True, False])
+@pytest.mark.parametrize("shape,lx,lz", SHAPES)
+def test_nonlinear_dual(shape, lx, lz, linearise):
+    g, d = pair(shape, lx, lz)
+    fwd = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=linearise)
+    plan = StepPlan(g, 3000.0, 0.01, KIND_DUAL, linearise=linearise, ops=fwd.ops, explicit_dt=-0.01, plan_re=-3000.0)
+    v = perturbation_field(g, seed=5, energy=1e-2)
+    uf = perturbation_field(g, seed=6, energy=1e-2)
+    hv, hg, conv = plan.nonlinear(v.to(DEV), None if linearise else uf.to(DEV))
+    base = O.base_flow_hat(d)
+    rv, rg, _, rc = O.nl_dual_skew(d, v, base, uf, linearise)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
+@pytest.mark.parametrize("shape,lx,lz", SHAPES)
+def test_step_ns(shape, lx, lz):
+    g, d = pair(shape, lx, lz)
+    re, dt = 5000.0, 0.005
+    dpdx = -2.0 / re
+    plan = StepPlan(g, re, dt, KIND_NS)
+    u = ns_field(g, seed=7)
+    sp = O.make_ns(d, re, dt, dpdx)
+    ref = u.clone()
+    out = u.to(DEV).clone()
+    for n in range(3):
+        ref = O.rk_step(sp, ref, n)
+        plan.step(out, dpdx, 1)
+        err = rel(out, ref)
+        print("step", n, "rel", err)
+        assert err < 1e-11 * (n + 1)
+
+
+@pytest.mark.parametrize("linearise", [True, False])
+def test_step_perturbation(linearise):
+    g, d = pair((24, 65, 4))
+    re, dt = 3000.0, 0.01
+    plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=linearise)
+    u = perturbation_field(g, seed=0, energy=1e-6 if linearise else 1e-2)
+    sp = O.make_perturbation(d, re, dt, linearise)
+    ref = u.clone()
+    out = u.to(DEV).clone()
+    for n in range(5):
+        ref = O.rk_step(sp, ref, n)
+        plan.step(out, 0.0, 1)
+        err = rel(out, ref)
+        print("step", n, "rel", err)
+        assert err < 1e-11 * (n + 1)
