@@ -98,6 +98,15 @@ int jsd_energy(jsd_plan* plan, const void* vel_hat, double* out, void* stream);
  * (FourierDomain.diff along y, domain.py:903-917). */
 int jsd_ydiff(jsd_plan* plan, const void* in, void* out, int ncomp, int order, void* stream);
 
+/* Instrumentation, no counterpart in the reference (its only timer is the
+ * wall clock around solve_scan, equation.py:347-401): with enable != 0 every
+ * later kernel launch is bracketed by HIP events on its stream; a call with
+ * ms / count != NULL first returns the accumulated milliseconds and launch
+ * counts of the 7 kernel classes (y-derivative GEMM, inverse x FFT, z
+ * products pass, forward x FFT, h_v GEMM, implicit solve, linearised term),
+ * then resets them. */
+int jsd_profile(jsd_plan* plan, int enable, double* ms, long* count);
+
 const char* jsd_last_error(void);
 int jsd_version(void);
 

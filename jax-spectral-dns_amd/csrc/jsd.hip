@@ -30,6 +30,8 @@
 
 #include "../../include/jsd.h"
 
+#define JSD_NPROF 7  // kernel classes: ygemm, fft_x_inv, fft_z_mid, fft_x_fwd, ygemm_hv, solve, nl_lin
+
 // ===========================================================================
 // y-derivative GEMM on FP64 MFMA
 // ===========================================================================
@@ -761,7 +763,32 @@ struct jsd_plan {
   double* partial;
   double* eout;
   int npartial;
+  // optional per-kernel-class timing with HIP events on the launch stream
+  int prof_on;
+  std::vector<hipEvent_t> prof_ev[JSD_NPROF];
+  size_t prof_used[JSD_NPROF];
 };
+
+static void prof_begin(jsd_plan* p, int cls, hipStream_t st) {
+  if (!p->prof_on) return;
+  auto& v = p->prof_ev[cls];
+  const size_t k = p->prof_used[cls];
+  if (v.size() < k + 2) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    v.push_back(a);
+    v.push_back(b);
+  }
+  (void)hipEventRecord(v[k], st);
+}
+static void prof_end(jsd_plan* p, int cls, hipStream_t st) {
+  if (!p->prof_on) return;
+  auto& v = p->prof_ev[cls];
+  const size_t k = p->prof_used[cls];
+  if (v.size() < k + 2) return;
+  (void)hipEventRecord(v[k + 1], st);
+  p->prof_used[cls] = k + 2;
+}
 
 static int dalloc(jsd_plan* p, void** ptr, size_t bytes) {
   hipError_t e = hipMalloc(ptr, bytes > 0 ? bytes : 16);
@@ -848,6 +875,8 @@ extern "C" const char* jsd_last_error(void) { return g_err.c_str(); }
 
 extern "C" int jsd_plan_destroy(jsd_plan* p) {
   if (!p) return JSD_OK;
+  for (int c = 0; c < JSD_NPROF; ++c)
+    for (hipEvent_t e : p->prof_ev[c]) (void)hipEventDestroy(e);
   for (void* a : p->allocs) (void)hipFree(a);
   delete p;
   return JSD_OK;
@@ -968,9 +997,11 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
 // ---------------------------------------------------------------------------
 static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int ncomp, int order, hipStream_t st) {
   dim3 grid(p->M, (2 * p->Z + 63) / 64, ncomp);
+  prof_begin(p, 0, st);
   hipLaunchKernelGGL(k_ygemm<GEMM_PLAIN>, grid, dim3(256), 0, st, order == 2 ? p->DF2 : p->DF1, p->NT, p->KS,
                      (const double*)in, nullptr, nullptr, (double*)out, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
                      p->Z, (long)(2 * p->fs));
+  prof_end(p, 0, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
 }
@@ -978,8 +1009,10 @@ static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int 
 static int launch_hv(jsd_plan* p, const double2* H, double2* hv, double2* hg, double2* conv, hipStream_t st) {
   dim3 grid(p->M, (2 * p->Z + 63) / 64, 1);
   const double* h = (const double*)H;
+  prof_begin(p, 4, st);
   hipLaunchKernelGGL(k_ygemm<GEMM_HV>, grid, dim3(256), 0, st, p->DF1, p->NT, p->KS, h, h + 2 * p->fs,
                      h + 4 * p->fs, (double*)hv, (double*)hg, (double*)conv, p->kx, p->kz, p->M, p->N, p->Z, 0L);
+  prof_end(p, 4, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
 }
@@ -994,8 +1027,10 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     int rc = launch_ygemm_plain(p, U + fs, p->DU, 1, 1, st);
     if (rc) return rc;
     const size_t tot = fs;
+    prof_begin(p, 6, st);
     hipLaunchKernelGGL(k_nl_lin, dim3((tot + 255) / 256), dim3(256), 0, st, U, (long)fs, p->DU, p->H, p->kx,
                        p->kz, p->Ub, p->dUb, M, N, Z, d.kind == JSD_KIND_DUAL ? 1 : 0);
+    prof_end(p, 6, st);
     HIPCHK(hipGetLastError());
     return JSD_OK;
   }
@@ -1005,12 +1040,14 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   const size_t ts = (size_t)d.ppx * N * Z;
   const size_t lds_x = (size_t)d.ppx * NSLOT * sizeof(double2);
   int ntr = 6;
+  prof_begin(p, 1, st);
   for (int c = 0; c < 3; ++c) {
     hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, U + c * fs, (int)OP_ID, U + c * fs,
                        (int)OP_DX, p->T + (2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
     hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, p->DU + c * fs, (int)OP_ID, U + c * fs,
                        (int)OP_DZ, p->T + (2 * c + 1) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
   }
+  prof_end(p, 1, st);
   int prod = PROD_NS;
   double clin = 1.0;
   if (d.kind == JSD_KIND_PERTURBATION) prod = PROD_PERT;
@@ -1030,11 +1067,15 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   }
   HIPCHK(hipGetLastError());
   const size_t lds_z = ((size_t)d.ppz * ntr + (size_t)d.ppz * 4) * sizeof(double2);
+  prof_begin(p, 2, st);
   hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, N), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V, (long)ts, p->fpz,
                      N, Z, prod, clin, p->Ub, p->dUb);
+  prof_end(p, 2, st);
   HIPCHK(hipGetLastError());
+  prof_begin(p, 3, st);
   hipLaunchKernelGGL(k_fft_x_fwd, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx, M,
                      N, Z, p->s_fwd);
+  prof_end(p, 3, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
 }
@@ -1083,7 +1124,9 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   S.dpdx = dPdx * p->dpdx_scale;
   S.use_old = s > 0;
   const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  prof_begin(p, 5, st);
   hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, S);
+  prof_end(p, 5, st);
   HIPCHK(hipGetLastError());
   std::swap(p->HV, p->HVo);
   std::swap(p->HG, p->HGo);
@@ -1152,4 +1195,25 @@ extern "C" int jsd_energy(jsd_plan* p, const void* vel_hat, double* out, void* s
 extern "C" int jsd_ydiff(jsd_plan* p, const void* in, void* out, int ncomp, int order, void* stream) {
   if (!p || !in || !out || ncomp < 1 || (order != 1 && order != 2)) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   return launch_ygemm_plain(p, (const double2*)in, (double2*)out, ncomp, order, (hipStream_t)stream);
+}
+
+extern "C" int jsd_profile(jsd_plan* p, int enable, double* ms, long* count) {
+  if (!p) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (ms || count) {
+    for (int c = 0; c < JSD_NPROF; ++c) {
+      double tot = 0.0;
+      const size_t used = p->prof_used[c];
+      for (size_t k = 0; k + 1 < used; k += 2) {
+        HIPCHK(hipEventSynchronize(p->prof_ev[c][k + 1]));
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, p->prof_ev[c][k], p->prof_ev[c][k + 1]));
+        tot += t;
+      }
+      if (ms) ms[c] = tot;
+      if (count) count[c] = (long)(used / 2);
+    }
+  }
+  for (int c = 0; c < JSD_NPROF; ++c) p->prof_used[c] = 0;
+  p->prof_on = enable ? 1 : 0;
+  return JSD_OK;
 }

@@ -4,7 +4,9 @@ The generator draws Fourier-Chebyshev coefficients for the wall-normal
 velocity v and vorticity omega_y with amplitude exp(-(|kx_i|+|kz_i|)/2) 0.8^n
 (kx_i, kz_i integer mode indices, n the Chebyshev degree), makes the spectrum
 Hermitian, evaluates it at the Gauss-Lobatto points, multiplies v by
-(1-y^2)^2 and omega_y by (1-y^2) so that v = v' = omega_y = 0 at the walls,
+(1-y^2)^2 and omega_y by (1-y^2) so that v = v' = omega_y = 0 at the walls (v uses degrees
+n <= Ny-5 and omega_y degrees n <= Ny-3, so the multiplied profiles stay in
+the Ny-point collocation space and the wall conditions hold to rounding),
 recovers u and w from continuity (the same relations as the per-wavenumber
 step, navier_stokes.py:1506-1527) and scales the field to a target energy.
 Output layout is the reference's: complex128 (3, Nx_modes, Ny, Nz_modes).
@@ -58,6 +60,8 @@ def perturbation(grid: Grid, seed: int = 0, energy: float = 1e-6, decay: float =
         return (re + 1j * im) / np.sqrt(2.0) * amp[:, :, None] * cheb[None, None, :]
 
     cv, cw = _hermitian(draw()), _hermitian(draw())
+    cv[:, :, ny - 4 :] = 0.0
+    cw[:, :, ny - 2 :] = 0.0
     cv[0, 0, :] = 0.0
     T = np.cos(np.outer(np.arange(ny), np.arange(ny)) * np.pi / (ny - 1))  # T_n(y_j) [j, n]
     y = grid.y

@@ -487,6 +487,18 @@ class Operators:
     lhs_inv_ns: np.ndarray  # (3, 2Ny, 2Ny)
 
 
+def _hom_dir_stack(mat: np.ndarray) -> np.ndarray:
+    """hom_dirichlet applied to every matrix of a stack (domain.py:375-397)."""
+    out = mat.copy()
+    out[..., 0, :] = 0.0
+    out[..., :, 0] = 0.0
+    out[..., -1, :] = 0.0
+    out[..., :, -1] = 0.0
+    out[..., 0, 0] = 1.0
+    out[..., -1, -1] = 1.0
+    return out
+
+
 def build_operators(dom: Dom, re_tau: float, dt: float) -> Operators:
     """prepare_assemble_rk_matrices (navier_stokes.py:920-1022) and
     assemble_poisson_matrix (domain.py:857-889), restricted to |kx|, |kz|."""
@@ -504,17 +516,21 @@ def build_operators(dom: Dom, re_tau: float, dt: float) -> Operators:
     rhs_inh = np.zeros((3, ka, kb, n))
     rhs_ns = np.zeros((3, 2 * n, 2 * n))
     lhs_inv_ns = np.zeros((3, 2 * n, 2 * n))
+    kx2 = dom.kx[:ka, None] ** 2
+    kz2 = dom.kz[None, :kb] ** 2
+    ksq = (kx2 + kz2)[:, :, None, None]  # kx**2 + kz**2 per (a, b), as in the loop body
+    L = Ly[None, None] + I[None, None] * (-(ksq)) / re_tau
     for s in range(3):
-        for a in range(ka):
-            kx = dom.kx[a]
-            for b in range(kb):
-                kz = dom.kz[b]
-                L = Ly + I * (-(kx**2 + kz**2)) / re_tau
-                rhs[s, a, b] = hom_dirichlet(I + RK_ALPHA[s] * dt * L)
-                lhs_inv[s, a, b] = np.linalg.inv(hom_dirichlet(I - RK_BETA[s] * dt * L))
-                lhs_inh, r_inh = inhom_dirichlet(I - RK_BETA[s] * dt * L, np.zeros(n), 0.0, 1.0)
-                lhs_inv_inh[s, a, b] = np.linalg.inv(lhs_inh)
-                rhs_inh[s, a, b] = r_inh
+        # stacked over (a, b); np.linalg.inv runs the same LAPACK solve per matrix
+        rhs[s] = _hom_dir_stack(I + RK_ALPHA[s] * dt * L)
+        lhs_inv[s] = np.linalg.inv(_hom_dir_stack(I - RK_BETA[s] * dt * L))
+        inh = (I - RK_BETA[s] * dt * L).copy()
+        inh[..., 0, :] = 0.0
+        inh[..., 0, 0] = 1.0
+        inh[..., -1, :] = 0.0
+        inh[..., -1, -1] = 1.0
+        lhs_inv_inh[s] = np.linalg.inv(inh)
+        rhs_inh[s] = inhom_dirichlet(I, np.zeros(n), 0.0, 1.0)[1]
         I_ns = np.eye(2 * n)
         L_ns = L_ns_y + I_ns * (-(0**2 + 0**2)) / re_tau
         rhs_ns[s] = hom_dirichlet(I_ns + RK_ALPHA[s] * dt * L_ns)
@@ -523,10 +539,7 @@ def build_operators(dom: Dom, re_tau: float, dt: float) -> Operators:
     eye_bc = np.eye(n)
     eye_bc[0, 0] = 0.0
     eye_bc[-1, -1] = 0.0
-    poisson = np.zeros((ka, kb, n, n))
-    for a in range(ka):
-        for b in range(kb):
-            poisson[a, b] = np.linalg.inv((-(dom.kx[a] ** 2 + dom.kz[b] ** 2)) * eye_bc + D2h)
+    poisson = np.linalg.inv((-(ksq)) * eye_bc[None, None] + D2h[None, None])
     return Operators(re_tau, dt, rhs, lhs_inv, lhs_inv_inh, rhs_inh, poisson, rhs_ns, lhs_inv_ns)
 
 
@@ -538,14 +551,21 @@ def _abs_index(m: int) -> np.ndarray:
 
 def apply_k(dom: Dom, table: np.ndarray, x: torch.Tensor, key: str) -> torch.Tensor:
     """Per-pencil matvec table[|kx|,|kz|] @ x[kx, kz, :] for complex pencils
-    x of shape (Nxm, Nzm, Ny) with a real table (Ka, Kb, Ny, Ny)."""
-    ai = torch.from_numpy(_abs_index(dom.nxm))
-    bi = torch.from_numpy(_abs_index(dom.nzm))
+    x of shape (Nxm, Nzm, Ny) with a real table (Ka, Kb, Ny, Ny).  The four
+    sign quadrants (+-kx, +-kz) each map one-to-one onto the table; pencils on
+    kx = 0 or kz = 0 are computed twice with identical values."""
     T = dom.t_(key, table, F64)
-    M = T[ai][:, bi]  # (Nxm, Nzm, Ny, Ny)
-    re = torch.matmul(M, x.real.unsqueeze(-1)).squeeze(-1)
-    im = torch.matmul(M, x.imag.unsqueeze(-1)).squeeze(-1)
-    return torch.complex(re, im)
+    ka, kb = T.shape[0], T.shape[1]
+    out = torch.empty_like(x)
+    a = torch.arange(ka)
+    b = torch.arange(kb)
+    for ia in (a, (dom.nxm - a) % dom.nxm):
+        for ib in (b, (dom.nzm - b) % dom.nzm):
+            xs = x[ia][:, ib]
+            re = torch.matmul(T, xs.real.unsqueeze(-1)).squeeze(-1)
+            im = torch.matmul(T, xs.imag.unsqueeze(-1)).squeeze(-1)
+            out[ia[:, None], ib[None, :]] = torch.complex(re, im)
+    return out
 
 
 def apply_k_vec(dom: Dom, table: np.ndarray, key: str) -> torch.Tensor:

@@ -1,0 +1,144 @@
+"""CPU-only checks of the product's host side: the C-ABI library loads and
+exports every symbol include/jsd.h declares (no compute call without a GPU),
+the host precompute matches the oracle's operator assembly, the transform
+plans are valid, and the seeded generator has the documented properties."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from jax_spectral_dns_amd import _lib
+from jax_spectral_dns_amd import domain_math as DM
+from jax_spectral_dns_amd import initial
+from oracle import jsd_oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_header_symbol():
+    header = open(os.path.join(REPO, "include", "jsd.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char\*|void)\s+(jsd_\w+)\s*\(", header, re.M))
+    assert declared == set(_lib.EXPORTS)
+    lib = _lib.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.jsd_version() >= 1
+
+
+def test_library_rejects_bad_plans_without_gpu():
+    """Argument validation happens before any device call."""
+    lib = _lib.load()
+    desc = _lib.PlanDesc()
+    desc.nxm, desc.ny, desc.nzm = 24, 65, 5  # even mode count
+    h = ctypes.c_void_p()
+    assert lib.jsd_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1
+    assert b"odd" in lib.jsd_last_error()
+    desc.nxm, desc.ny = 25, 400
+    assert lib.jsd_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1
+    assert lib.jsd_plan_create(None, ctypes.byref(h)) == -1
+
+
+@pytest.mark.parametrize("shape,lx,lz", [((24, 65, 4), 2 * np.pi, 2 * np.pi), ((128, 129, 128), 4 * np.pi, 2 * np.pi)])
+def test_grid_matches_oracle(shape, lx, lz):
+    g = DM.make_grid(shape, lx, lz)
+    d = O.make_domain(shape, lx, lz)
+    assert (g.nxm, g.ny, g.nzm, g.px, g.pz) == (d.nxm, d.ny, d.nzm, d.px, d.pz)
+    assert np.array_equal(g.y, d.y)
+    assert np.array_equal(g.kx, d.kx) and np.array_equal(g.kz, d.kz)
+    assert np.abs(g.D - d.D).max() <= 1e-13 * np.abs(d.D).max()
+    assert np.abs(g.y_weights - O.y_int_weights(d)).max() < 1e-14
+    assert g.scaling == d.scaling
+    assert g.ppx >= 3 * (g.nxm // 2) + 1 and DM.factorize(g.ppx)
+    assert g.ppz >= 3 * (g.nzm // 2) + 1 and DM.factorize(g.ppz)
+
+
+def test_operator_tables_match_oracle():
+    shape = (24, 33, 8)
+    g = DM.make_grid(shape)
+    d = O.make_domain(shape)
+    ops = DM.build_operators(g, 1234.0, 0.013)
+    ref = O.build_operators(d, 1234.0, 0.013)
+    for s in range(3):
+        assert np.abs(ops.lhs_inv[s] - ref.lhs_inv[s]).max() < 1e-12
+        # R_s = rhs_base + alpha dt (-k^2)/Re on the interior diagonal
+        for a in range(g.ka):
+            for b in range(g.kb):
+                k2 = g.kx[a] ** 2 + g.kz[b] ** 2
+                R = ops.rhs_base[s].copy()
+                idx = np.arange(1, g.ny - 1)
+                R[idx, idx] += DM.RK_ALPHA[s] * 0.013 * (-k2) / 1234.0
+                assert np.abs(R - ref.rhs[s, a, b]).max() < 1e-13
+    assert np.abs(ops.poisson - ref.poisson).max() < 1e-10 * np.abs(ref.poisson).max()
+    assert np.abs(ops.rhs_ns - ref.rhs_ns).max() < 1e-15
+    assert np.abs(ops.lhs_inv_ns - ref.lhs_inv_ns).max() < 1e-12
+
+
+def test_wall_correction_solution_satisfies_neumann():
+    """v_p + a v_a + b flip(v_a) has zero wall derivative for any v_p."""
+    g = DM.make_grid((24, 33, 8))
+    ops = DM.build_operators(g, 3000.0, 0.01)
+    rng = np.random.default_rng(0)
+    for s in range(3):
+        for a, b in ((0, 0), (3, 2), (g.ka - 1, g.kb - 1)):
+            vp = rng.standard_normal(g.ny)
+            vp[0] = vp[-1] = 0.0
+            dvp = g.D @ vp
+            mi = ops.infl_inv[s, a, b].reshape(2, 2)
+            ab = mi @ np.array([-dvp[0], -dvp[-1]])
+            va = ops.v_a[s, a, b]
+            v = vp + ab[0] * va + ab[1] * va[::-1]
+            dv = g.D @ v
+            assert abs(dv[0]) < 1e-8 * np.abs(dvp).max() and abs(dv[-1]) < 1e-8 * np.abs(dvp).max()
+
+
+@pytest.mark.parametrize("p", [7, 8, 25, 40, 49, 75, 125, 147, 196])
+def test_fft_plan_is_a_permutation_and_exact(p):
+    perm, tw, rad = DM.dit_tables(p, inverse=False)
+    assert sorted(perm.tolist()) == list(range(p))
+    assert int(np.prod(rad)) == p
+    x = np.random.default_rng(p).standard_normal(p) + 1j
+    # execute the in-place DIT plan exactly as the kernel does
+    buf = x[perm].astype(np.complex128)
+    L = 1
+    for r in rad:
+        Lp, L = L, L * r
+        for g0 in range(p // L):
+            for k1 in range(Lp):
+                pos = g0 * L + k1 + np.arange(r) * Lp
+                v = buf[pos] * tw[(np.arange(r) * k1 * (p // L)) % p]
+                buf[pos] = np.array([sum(v[m] * tw[(q * m * (p // r)) % p] for m in range(r)) for q in range(r)])
+    assert np.abs(buf - np.fft.fft(x)).max() < 1e-13 * p
+
+
+def test_generator_properties():
+    g = DM.make_grid((24, 33, 16), 2 * np.pi, np.pi)
+    u = initial.perturbation(g, seed=5, energy=1e-6)
+    assert u.shape == (3, g.nxm, g.ny, g.nzm)
+    assert abs(initial.spectral_energy(g, u) - 1e-6) < 1e-18
+    # walls: u = v = w = 0 and dv/dy = 0
+    assert np.abs(u[:, :, 0]).max() < 1e-14 and np.abs(u[:, :, -1]).max() < 1e-14
+    dv = np.einsum("lj,ijk->ilk", g.D, u[1])
+    assert np.abs(dv[:, 0]).max() < 1e-10 * np.abs(u[1]).max() * g.ny**2
+    # Hermitian: u[-k] = conj u[k]
+    pi = (g.nxm - np.arange(g.nxm)) % g.nxm
+    pj = (g.nzm - np.arange(g.nzm)) % g.nzm
+    assert np.abs(u - np.conj(u[:, pi][:, :, :, pj])).max() < 1e-15
+    # solenoidal: i kx u + D v + i kz w = 0
+    div = 1j * g.kx[:, None, None] * u[0] + dv + 1j * g.kz[None, None, :] * u[2]
+    assert np.abs(div).max() < 1e-10 * np.abs(u).max() * g.ny**2
+    # deterministic
+    assert np.array_equal(u, initial.perturbation(g, seed=5, energy=1e-6))
+
+
+def test_product_requires_gpu():
+    """Without a GPU the plan refuses to build: no CPU fallback exists."""
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from jax_spectral_dns_amd.plan import StepPlan
+
+    with pytest.raises(_lib.JsdError):
+        StepPlan(DM.make_grid((24, 65, 4)), 3000.0, 0.01, _lib.KIND_PERTURBATION)
