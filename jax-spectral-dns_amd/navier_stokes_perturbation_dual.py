@@ -1,0 +1,171 @@
+"""NavierStokesVelVortPerturbationDual: the continuous adjoint of the
+perturbation step (reference navier_stokes_perturbation_dual.py:376-1212).
+
+The dual is the same RK3/CN stepper with Re -> -Re and dt -> -dt, reusing the
+forward operator tables (dt L is unchanged; navier_stokes_perturbation_dual.py:
+470-471, :542-589), with the adjoint nonlinear term linearised about the
+forward state history[-1-t] (navier_stokes_perturbation_dual.py:591-602).
+The forward trajectory is kept on the GPU either whole or, with
+checkpointing, every ``inner`` steps and recomputed chunk by chunk before
+each adjoint sweep (navier_stokes_perturbation_dual.py:690-757, 990-1035).
+The gain gradient is (u0 - v0) / E0 (navier_stokes_perturbation_dual.py:
+1106-1120)."""
+
+from __future__ import annotations
+
+import math
+from typing import Any, List, Optional, Tuple
+
+import torch
+
+from . import _lib
+from .field import FourierField, VectorField
+from .navier_stokes import median_factor
+from .navier_stokes_perturbation import NavierStokesVelVortPerturbation
+from .plan import StepPlan
+
+
+class NavierStokesVelVortPerturbationDual:
+    name = "Dual Navier Stokes equation (velocity-vorticity formulation) for perturbations on top of a base flow."
+
+    def __init__(self, forward_equation: NavierStokesVelVortPerturbation, **params: Any):
+        self.forward_equation = forward_equation
+        mode = params.get("optimisation_mode", "gain")
+        if mode != "gain":
+            raise NotImplementedError("optimisation modes other than 'gain' are not part of the MI355X step")
+        self.optimisation_mode = mode
+        fwd = forward_equation
+        self.Re_tau = -fwd.get_Re_tau()
+        self.dt = -fwd.get_dt()
+        self.end_time = -fwd.end_time
+        self.linearise_flag = fwd.linearise()
+        n = fwd.number_of_time_steps()
+        calc = abs(self.end_time / self.dt)
+        for s in fwd.physical_domain.get_shape():
+            calc *= s
+        # checkpointing threshold of the reference (navier_stokes_perturbation_dual.py:420-434)
+        self.checkpointing = params.get("checkpointing", calc > params.get("checkpointing_threshold", 1.4e8))
+        self.number_of_time_steps = n
+        self.number_of_inner_steps = params.get("checkpoint_every", median_factor(n) if n > 0 else 1)
+        if n % self.number_of_inner_steps != 0:
+            raise ValueError("checkpoint interval must divide the number of steps")
+        self.number_of_outer_steps = n // self.number_of_inner_steps
+        fplan = fwd.plan()
+        self._plan = StepPlan(
+            fplan.grid,
+            fplan.re_tau,
+            fplan.dt,
+            _lib.KIND_DUAL,
+            self.linearise_flag,
+            fwd.u_max_over_u_tau,
+            ops=fplan.ops,
+            explicit_dt=self.dt,
+            plan_re=self.Re_tau,
+        )
+        self.velocity_u_hat_0: Optional[torch.Tensor] = None
+        self.u_T: Optional[torch.Tensor] = None
+        self.v_0: Optional[torch.Tensor] = None
+        self._checkpoints: List[torch.Tensor] = []
+        self._history: List[torch.Tensor] = []
+        self.gain: Optional[float] = None
+
+    @classmethod
+    def FromNavierStokesVelVortPerturbation(cls, nse: NavierStokesVelVortPerturbation, **params: Any):
+        return cls(nse, **params)
+
+    def plan(self) -> StepPlan:
+        return self._plan
+
+    # forward (navier_stokes_perturbation_dual.py:850-939)
+    def run_forward_calculation(self) -> None:
+        fwd = self.forward_equation
+        fplan = fwd.plan()
+        u = fwd._state(fwd.get_initial_field("velocity_hat").get_data()).clone()
+        self.velocity_u_hat_0 = u.clone()
+        n, inner = self.number_of_time_steps, self.number_of_inner_steps
+        need_history = not self.linearise_flag
+        self._history, self._checkpoints = [], []
+        if need_history and not self.checkpointing:
+            self._history.append(u.clone())
+            for _ in range(n):
+                fplan.step(u, 0.0, 1)
+                self._history.append(u.clone())
+        else:
+            self._checkpoints.append(u.clone())
+            for _ in range(self.number_of_outer_steps):
+                fplan.step(u, 0.0, inner)
+                if need_history:
+                    self._checkpoints.append(u.clone())
+        self.u_T = u
+
+    def _forward_state(self, t: int, chunk: List[torch.Tensor], start: int) -> Optional[torch.Tensor]:
+        if self.linearise_flag:
+            return None
+        if not self.checkpointing:
+            return self._history[-1 - t]
+        return chunk[-1 - (t - start)]
+
+    # backward (navier_stokes_perturbation_dual.py:690-839, 1037-1056)
+    def run_backward_calculation(self) -> torch.Tensor:
+        if self.u_T is None:
+            self.run_forward_calculation()
+        assert self.u_T is not None
+        fplan = self.forward_equation.plan()
+        v = -self.u_T.clone()  # adjoint initial condition for the gain (dual.py:904-907)
+        n, inner = self.number_of_time_steps, self.number_of_inner_steps
+        if self.linearise_flag or not self.checkpointing:
+            for t in range(n):
+                self._plan.dual_step(v, self._forward_state(t, [], 0))
+        else:
+            for o in range(self.number_of_outer_steps):
+                u = self._checkpoints[-1 - (o + 1)].clone()
+                chunk = [u.clone()]
+                for _ in range(inner):
+                    fplan.step(u, 0.0, 1)
+                    chunk.append(u.clone())
+                for t in range(o * inner, (o + 1) * inner):
+                    self._plan.dual_step(v, self._forward_state(t, chunk, o * inner))
+        self.v_0 = v
+        return v
+
+    def get_gain(self) -> float:
+        if self.u_T is None:
+            self.run_forward_calculation()
+        p = self.forward_equation.plan()
+        self.gain = p.energy(self.u_T) / p.energy(self.velocity_u_hat_0)
+        return self.gain
+
+    def get_objective_fun(self) -> float:
+        return self.get_gain()
+
+    def get_grad(self, u_hat_0: Optional[torch.Tensor] = None, v_hat_0: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """(u0 - v0) / E0 (navier_stokes_perturbation_dual.py:1106-1120)."""
+        if u_hat_0 is None and v_hat_0 is None:
+            self.run_backward_calculation()
+        u0 = self.velocity_u_hat_0 if u_hat_0 is None else u_hat_0
+        v0 = self.v_0 if v_hat_0 is None else v_hat_0
+        self.get_gain()
+        e0 = self.forward_equation.plan().energy(u0)
+        return (u0 - v0) / e0
+
+    def get_projected_grad(self, step_size: float, step_size_factor: Optional[float] = None) -> Tuple[torch.Tensor, bool]:
+        """Energy-constrained update direction (navier_stokes_perturbation_dual.py:1122-1167)."""
+        if self.v_0 is None:
+            self.run_backward_calculation()
+        f = 0.9 if step_size_factor is None else step_size_factor
+        p = self.forward_equation.plan()
+        u0, v0 = self.velocity_u_hat_0, self.v_0
+        e0 = p.energy(u0)
+        e0_adj = p.energy(v0)
+        c0 = 2 * 0.5 * (p.energy(u0 + v0) - e0 - e0_adj)  # 2 <u0, v0>_E
+        max_step = f * (e0_adj / e0 - c0**2 / (4 * e0**2)) ** (-1 / 2)
+        step_size = min(step_size, max_step)
+        A = (step_size * c0 - 2 * e0) / (2 * step_size * e0)
+        lam = A + math.sqrt(A**2 - (step_size * e0_adj - c0) / (step_size * e0))
+        return lam * u0 - v0, True
+
+
+def perform_step_navier_stokes_perturbation_dual(nse: NavierStokesVelVortPerturbation, step_size: float = 1e-2):
+    """navier_stokes_perturbation_dual.py:1204-1212."""
+    dual = NavierStokesVelVortPerturbationDual.FromNavierStokesVelVortPerturbation(nse)
+    return dual.get_gain(), dual.get_projected_grad(step_size)[0]

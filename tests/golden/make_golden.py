@@ -1,0 +1,55 @@
+"""Writes the golden fixtures under tests/golden/ by running the oracle (the
+CPU restatement of the reference step).  Run from the repository root:
+
+    python tests/golden/make_golden.py cfg0
+
+cfg0: linearised perturbation about U = 1 - y^2, Re = 3000, Lx = Lz = 2 pi,
+24 x 65 x 4 requested modes (25 x 65 x 5 stored), dt = 0.01, 1500 steps;
+initial field: seeded generator, seed 0, energy 1e-6.  Stores the initial
+state, the states after 1, 10 and 1500 steps and the energy history.
+"""
+
+import os
+import sys
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from jax_spectral_dns_amd import domain_math as DM  # noqa: E402
+from jax_spectral_dns_amd import initial  # noqa: E402
+from oracle import jsd_oracle as O  # noqa: E402
+
+
+def cfg0() -> None:
+    torch.set_num_threads(1)
+    shape, re, dt, n = (24, 65, 4), 3000.0, 0.01, O.number_of_steps(15.0, 0.01)
+    g = DM.make_grid(shape)
+    d = O.make_domain(shape)
+    u0 = torch.from_numpy(initial.perturbation(g, seed=0, energy=1e-6))
+    sp = O.make_perturbation(d, re, dt, linearise=True)
+    u = u0.clone()
+    energy = [float(O.energy(d, u))]
+    keep = {}
+    for t in range(n):
+        u = O.rk_step(sp, u, t)
+        energy.append(float(O.energy(d, u)))
+        if t + 1 in (1, 10):
+            keep["u%d" % (t + 1)] = u.numpy().copy()
+    np.savez_compressed(
+        os.path.join(HERE, "cfg0_linearised_1500.npz"),
+        u0=u0.numpy(),
+        u1=keep["u1"],
+        u10=keep["u10"],
+        uT=u.numpy(),
+        energy=np.array(energy),
+        params=np.array([re, dt, n, 1e-6, 0.0]),
+    )
+    print("cfg0: n=%d gain E(T)/E(0) = %.12f" % (n, energy[-1] / energy[0]))
+
+
+if __name__ == "__main__":
+    for name in sys.argv[1:] or ["cfg0"]:
+        globals()[name]()

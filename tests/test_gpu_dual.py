@@ -1,0 +1,72 @@
+"""Dual (continuous adjoint) step and gain gradient on the GPU against the
+oracle's adjoint driver (navier_stokes_perturbation_dual.py:690-1120), plus
+the checkpoint-and-recompute identity of the reference's test_checkpointing
+(tests/test_project.py:1505-1555)."""
+
+import numpy as np
+import pytest
+import torch
+
+from jax_spectral_dns_amd.domain import PhysicalDomain
+from jax_spectral_dns_amd.field import FourierField, VectorField
+from jax_spectral_dns_amd.navier_stokes_perturbation import NavierStokesVelVortPerturbation
+from jax_spectral_dns_amd.navier_stokes_perturbation_dual import NavierStokesVelVortPerturbationDual
+from oracle import jsd_oracle as O
+from tests.helpers import pair, perturbation_field, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _nse(shape, lx, lz, re, dt, T, linearise, u0):
+    pd = PhysicalDomain.create(shape, (True, False, True), scale_factors=(lx, 1.0, lz))
+    vh = VectorField.FromData(FourierField, pd, u0.to("cuda"), "velocity_hat")
+    return NavierStokesVelVortPerturbation(vh, Re=re, dt=dt, end_time=T, linearise=linearise)
+
+
+@pytest.mark.parametrize("linearise,shape,lx,lz,n", [(True, (24, 65, 4), 2 * np.pi, 2 * np.pi, 12), (False, (24, 33, 16), 2 * np.pi, np.pi, 6)])
+def test_dual_gradient_matches_oracle(linearise, shape, lx, lz, n):
+    re, dt = 2000.0, 0.01
+    g, d = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=8, energy=1e-6 if linearise else 1e-3)
+    nse = _nse(shape, lx, lz, re, dt, n * dt, linearise, u0)
+    dual = NavierStokesVelVortPerturbationDual.FromNavierStokesVelVortPerturbation(nse, checkpointing=False)
+    grad = dual.get_grad()
+    gain = dual.gain
+    fwd = O.make_perturbation(d, re, dt, linearise)
+    rg, rgain, _ = O.dual_gradient(fwd, u0, n, linearise)
+    print("dual grad rel %.3e gain %.15f vs %.15f" % (rel(grad, rg), gain, float(rgain)))
+    assert rel(grad, rg) < 1e-10
+    assert abs(gain - float(rgain)) / float(rgain) < 1e-11
+
+
+def test_checkpointed_dual_identical_on_gpu():
+    shape, lx, lz, re, dt, n = (24, 33, 16), 2 * np.pi, np.pi, 1000.0, 0.02, 6
+    g, d = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=1, energy=1e-3)
+    a = NavierStokesVelVortPerturbationDual(_nse(shape, lx, lz, re, dt, n * dt, False, u0), checkpointing=False)
+    b = NavierStokesVelVortPerturbationDual(_nse(shape, lx, lz, re, dt, n * dt, False, u0), checkpointing=True, checkpoint_every=3)
+    va = a.run_backward_calculation()
+    vb = b.run_backward_calculation()
+    diff = va - vb
+    e = a.forward_equation.plan().energy(diff.contiguous())
+    print("checkpointed vs stored adjoint: energy of difference %.3e" % e)
+    assert e < 1e-30
+    assert torch.equal(va, vb)
+
+
+def test_gradient_descent_step_increases_gain():
+    """test_dual (tests/test_project.py:1557-1581): one projected-gradient
+    step at fixed energy increases the linear transient-growth gain."""
+    shape, lx, lz, re, dt, n = (24, 65, 4), 2 * np.pi, 2 * np.pi, 3000.0, 0.01, 100
+    g, _ = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=0, energy=1e-6)
+    dual = NavierStokesVelVortPerturbationDual(_nse(shape, lx, lz, re, dt, n * dt, True, u0))
+    gain0 = dual.get_gain()
+    s = 1e-2
+    grad, _ = dual.get_projected_grad(s)
+    # gradient_descent_solver.py:398: next guess = current + step_size * grad
+    u1 = u0.to("cuda") + s * grad
+    dual2 = NavierStokesVelVortPerturbationDual(_nse(shape, lx, lz, re, dt, n * dt, True, u1.cpu()))
+    gain1 = dual2.get_gain()
+    print("gain %.6f -> %.6f" % (gain0, gain1))
+    assert gain1 > gain0
