@@ -94,6 +94,18 @@ int jsd_dual_step(jsd_plan* plan, void* adj_hat, const void* fwd_hat, void* stre
  * (field.py:607-611, field.py:1468-1473): writes the energy to *out (host). */
 int jsd_energy(jsd_plan* plan, const void* vel_hat, double* out, void* stream);
 
+/* Discrete adjoint (vector-Jacobian product) of nsteps linearised
+ * perturbation steps: grad holds the adjoint of the state after the steps on
+ * entry and the adjoint of the state before them on exit (real-pair view of
+ * the complex128 layout).  Replaces reverse-mode AD through the scan,
+ * jax.value_and_grad in optimiser.py:115-121 (navier_stokes.py:1839-1851).
+ * Only for JSD_KIND_PERTURBATION plans with linearise = 1. */
+int jsd_step_vjp(jsd_plan* plan, void* grad, int nsteps, void* stream);
+
+/* out = scale * dE/d(vel_hat) on the real-pair view, E the energy of
+ * jsd_energy (field.py:607-611). */
+int jsd_energy_grad(jsd_plan* plan, const void* vel_hat, void* out, double scale, void* stream);
+
 /* y-derivative of order 1 or 2 of ncomp spectral components
  * (FourierDomain.diff along y, domain.py:903-917). */
 int jsd_ydiff(jsd_plan* plan, const void* in, void* out, int ncomp, int order, void* stream);

@@ -65,6 +65,8 @@ EXPORTS = (
     "jsd_dual_step",
     "jsd_energy",
     "jsd_ydiff",
+    "jsd_step_vjp",
+    "jsd_energy_grad",
     "jsd_profile",
     "jsd_last_error",
     "jsd_version",
@@ -86,6 +88,8 @@ def load() -> ctypes.CDLL:
     lib.jsd_dual_step.argtypes = [vp, vp, vp, vp]
     lib.jsd_energy.argtypes = [vp, vp, _dp, vp]
     lib.jsd_ydiff.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
+    lib.jsd_step_vjp.argtypes = [vp, vp, ctypes.c_int, vp]
+    lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
     lib.jsd_last_error.restype = ctypes.c_char_p
     lib.jsd_version.restype = ctypes.c_int

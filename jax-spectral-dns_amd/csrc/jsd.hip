@@ -25,6 +25,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -40,7 +41,8 @@
 // per MFMA; the wave owns one 16-column strip and all row tiles; the WG
 // (4 waves) owns 64 columns.  D^o is pre-tiled on the host into MFMA A
 // fragments: DF[(lt*KS + ks)*64 + lane] = D^o[lt*16 + lane%16][ks*4 + lane/16].
-enum { GEMM_PLAIN = 0, GEMM_HV = 1 };
+enum { GEMM_PLAIN = 0, GEMM_HV = 1, GEMM_HVT = 2, GEMM_ACC = 3 };
+#define GEMM_GK 8  // k-steps per LDS-staged chunk of A fragments
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, int NT, int KS,
@@ -49,6 +51,7 @@ __global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, in
                                                const double* __restrict__ in2,  // HV: H2
                                                double* __restrict__ out0,       // PLAIN: out; HV: h_v
                                                double* __restrict__ out1,       // HV: h_g
+                                               double* __restrict__ out2,       // HVT: third output
                                                double* __restrict__ conv,       // HV: conv lines (2*N complex)
                                                const double* __restrict__ kx, const double* __restrict__ kz, int M,
                                                int N, int Z, long comp_stride) {
@@ -74,25 +77,80 @@ __global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, in
     const int j = ks * 4 + kr;
     if (!colok || j >= N) return 0.0;
     const size_t e = rowbase + (size_t)j * C2;
-    if (MODE == GEMM_PLAIN) return in0[e + col];
+    if (MODE != GEMM_HV) return in0[e + col];
     // X = (i kx) H0 + (i kz) H2 ; (i k)(a + ib) = (-k b, k a)
     const size_t c0 = e + (kzi << 1);
     if (par == 0) return (-kxv * in0[c0 + 1]) + (-kzv * in2[c0 + 1]);
     return (kxv * in0[c0]) + (kzv * in2[c0]);
   };
 
-  double bnext = loadB(0);
-  for (int ks = 0; ks < KS; ++ks) {
-    const double b = bnext;
-    if (ks + 1 < KS) bnext = loadB(ks + 1);
-    const double* A = DF + (size_t)ks * 64 + lane;
+  // A fragments of GK consecutive k-steps for all row tiles are staged in LDS
+  // once per workgroup (shared by its 4 waves); B values of the next chunk
+  // are loaded while the current chunk's MFMAs run.
+  __shared__ double sA[JSD_NTMAX * GEMM_GK * 64];
+  double bcur[GEMM_GK], bnext[GEMM_GK];
 #pragma unroll
-    for (int lt = 0; lt < JSD_NTMAX; ++lt) {
-      if (lt < NT) {
-        const double a = A[(size_t)lt * KS * 64];
-        acc[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[lt], 0, 0, 0);
+  for (int kk = 0; kk < GEMM_GK; ++kk) bcur[kk] = loadB(kk);
+  for (int kc = 0; kc < KS; kc += GEMM_GK) {
+    __syncthreads();
+    for (int it = threadIdx.x; it < NT * GEMM_GK * 64; it += blockDim.x) {
+      const int lt = it / (GEMM_GK * 64);
+      const int rem = it - lt * (GEMM_GK * 64);
+      const int kk = rem >> 6;
+      sA[it] = (kc + kk < KS) ? DF[((size_t)lt * KS + kc + kk) * 64 + (rem & 63)] : 0.0;
+    }
+#pragma unroll
+    for (int kk = 0; kk < GEMM_GK; ++kk) bnext[kk] = loadB(kc + GEMM_GK + kk);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GEMM_GK; ++kk) {
+#pragma unroll
+      for (int lt = 0; lt < JSD_NTMAX; ++lt) {
+        if (lt < NT) {
+          const double a = sA[(lt * GEMM_GK + kk) * 64 + lane];
+          acc[lt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur[kk], acc[lt], 0, 0, 0);
+        }
       }
     }
+#pragma unroll
+    for (int kk = 0; kk < GEMM_GK; ++kk) bcur[kk] = bnext[kk];
+  }
+  if (MODE == GEMM_HVT) {
+    // transpose of the h_v / h_g / conv assembly: acc = D^T hvbar, Xbar = -acc;
+    // Hbar0 = conj(i kx) Xbar + conj(i kz) hgbar (- convbar0), Hbar1 = (-kx^2 - kz^2) hvbar,
+    // Hbar2 = conj(i kz) Xbar + conj(-i kx) hgbar (- convbar2).  The other
+    // component of Xbar sits in the neighbouring lane (column col ^ 1).
+#pragma unroll
+    for (int lt = 0; lt < JSD_NTMAX; ++lt) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double own = -acc[lt][r];
+        const double oth = -__shfl_xor(acc[lt][r], 1);
+        const int l = lt * 16 + r * 4 + kr;
+        if (lt >= NT || l >= N || !colok) continue;
+        const double xr = par == 0 ? own : oth, xi = par == 0 ? oth : own;
+        const size_t e = rowbase + (size_t)l * C2;
+        const size_t c0 = e + (kzi << 1);
+        const double gr = in1[c0], gi = in1[c0 + 1];
+        double h0, h2;
+        if (par == 0) {
+          h0 = kxv * xi + kzv * gi;
+          h2 = kzv * xi - kxv * gi;
+        } else {
+          h0 = -kxv * xr - kzv * gr;
+          h2 = -kzv * xr + kxv * gr;
+        }
+        if (i == 0 && kzi == 0) {
+          h0 -= conv[2 * l + par];
+          h2 -= conv[2 * (N + l) + par];
+        }
+        const double hv = in0[e + col];
+        out0[e + col] = h0;
+        out1[e + col] = (-(kxv * kxv)) * hv + (-(kzv * kzv)) * hv;
+        out2[e + col] = h2;
+      }
+    }
+    return;
   }
   if (!colok) return;
 #pragma unroll
@@ -104,6 +162,8 @@ __global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, in
       const size_t e = rowbase + (size_t)l * C2;
       if (MODE == GEMM_PLAIN) {
         out0[e + col] = acc[lt][r];
+      } else if (MODE == GEMM_ACC) {
+        out0[e + col] += acc[lt][r];
       } else {
         // h_v = (-D X + (-kx^2) H1) + (-kz^2) H1   (navier_stokes.py:53-61)
         const double h1 = in1[e + col];
@@ -153,16 +213,27 @@ __device__ __forceinline__ double2 apply_op(int op, double2 v, double kxv, doubl
 }
 
 // Inverse x pass: T[x][l][j] = sum_kx (Herm(opA A) + i Herm(opB B))[kx, l, j] e^{+i kx x}
-// scaled by s_inv.  grid (nqb, N), block 256.
-__global__ __launch_bounds__(256) void k_fft_x_inv(const double2* __restrict__ A, int opA,
-                                                   const double2* __restrict__ B, int opB,
-                                                   double2* __restrict__ T, FftPlan PX, const double* __restrict__ kx,
-                                                   const double* __restrict__ kz, int M, int N, int Z,
-                                                   double s_inv) {
+// scaled by s_inv.  One launch covers all packed transforms (blockIdx.z).
+// grid (nqb, N, ntrans), block 256.
+#define JSD_MAXTR 12
+struct XInvSet {
+  const double2* A[JSD_MAXTR];
+  const double2* B[JSD_MAXTR];
+  double2* T[JSD_MAXTR];
+  int opA[JSD_MAXTR];
+  int opB[JSD_MAXTR];
+};
+
+__global__ __launch_bounds__(256) void k_fft_x_inv(XInvSet set, FftPlan PX, const double* __restrict__ kx,
+                                                   const double* __restrict__ kz, int M, int N, int Z, double s_inv) {
   extern __shared__ double2 lds[];
   double2* buf = lds;  // [p][slot]
   const int q0 = blockIdx.x * QB;
   const int l = blockIdx.y;
+  const int tr = blockIdx.z;
+  const double2* __restrict__ A = set.A[tr];
+  const double2* __restrict__ B = set.B[tr];
+  const int opA = set.opA[tr], opB = set.opB[tr];
   const int K = M / 2, KZ = Z / 2;
   const int P = PX.p;
   for (int w = threadIdx.x; w < P * NSLOT; w += blockDim.x) {
@@ -192,6 +263,7 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(const double2* __restrict__ A
   }
   __syncthreads();
   fft_lds(buf, NSLOT, NSLOT, 1, PX, +1.0);
+  double2* __restrict__ T = set.T[tr];
   for (int w = threadIdx.x; w < P * NSLOT; w += blockDim.x) {
     const int c = w % NSLOT;
     const int x = w / NSLOT;
@@ -205,52 +277,65 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(const double2* __restrict__ A
 enum { PROD_NS = 0, PROD_PERT = 1, PROD_DUAL = 2 };
 
 // Middle z pass: inverse z-FFT of nt packed inputs, products, forward z-FFT of
-// two packed outputs.  grid (PXlen, N), block 256.
+// two packed outputs, for YB consecutive y planes of one x.  grid
+// (PXlen, ceil(N/YB)), block 256.  LDS rows r = yb*nt + t hold element
+// (pos, r) at lds[pos*R + r]; after the products the same memory holds the
+// 2*YB output rows, written directly in DIT input order (iperm).
 // Input rows (NS / perturbation): row 2c = (u_c, d_x u_c), row 2c+1 = (d_y u_c, d_z u_c).
-// Dual adds rows 6+2c, 7+2c for the transport field W = c u_fwd (+ U on the host side).
+// Dual adds rows 6+2c, 7+2c for the forward field (the base flow is added here).
+#define ZMID_ITEMS 8
 __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T, long tstride, int nt,
                                                    double2* __restrict__ V, long vstride, FftPlan PZ, int N, int Z,
                                                    int prod, double clin, const double* __restrict__ Ub,
-                                                   const double* __restrict__ dUb) {
+                                                   const double* __restrict__ dUb, int YB) {
   extern __shared__ double2 lds[];
   const int P = PZ.p;
-  double2* buf = lds;             // [p][nt]
-  double2* obuf = lds + P * nt;   // [p][2]
   const int x = blockIdx.x;
-  const int l = blockIdx.y;
+  const int l0 = blockIdx.y * YB;
+  const int nyb = min(YB, N - l0);
   const int KZ = Z / 2;
-  for (int w = threadIdx.x; w < P * nt; w += blockDim.x) {
-    const int t = w % nt;
-    const int pos = w / nt;
-    const int n = PZ.perm[pos];
-    int j = -1;
-    if (n <= KZ) j = n;
-    else if (n >= P - KZ) j = n - P + Z;
-    buf[pos * nt + t] = (j >= 0) ? T[t * tstride + ((size_t)x * N + l) * Z + j] : make_double2(0.0, 0.0);
+  const int R = nt * YB;
+  // zero the padded band, then scatter the kept kz bins (read with consecutive
+  // lanes on consecutive kz) into their DIT positions
+  for (int w = threadIdx.x; w < P * R; w += blockDim.x) lds[w] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int w = threadIdx.x; w < Z * R; w += blockDim.x) {
+    const int j = w % Z;
+    const int r = w / Z;
+    const int yb = r / nt, t = r - yb * nt;
+    if (yb >= nyb) continue;
+    const int n = (j <= KZ) ? j : P - (Z - j);
+    lds[PZ.iperm[n] * R + r] = T[t * tstride + ((size_t)x * N + l0 + yb) * Z + j];
   }
   __syncthreads();
-  fft_lds(buf, nt, nt, 1, PZ, +1.0);
-  const double U = Ub ? Ub[l] : 0.0;
-  const double dU = dUb ? dUb[l] : 0.0;
-  for (int z = threadIdx.x; z < P; z += blockDim.x) {
+  fft_lds(lds, R, R, 1, PZ, +1.0);
+  double Hs[ZMID_ITEMS][3];
+#pragma unroll
+  for (int q = 0; q < ZMID_ITEMS; ++q) {
+    const int item = threadIdx.x + q * blockDim.x;
+    if (item >= P * YB) continue;
+    const int yb = item / P, z = item - yb * P;
+    const int l = min(l0 + yb, N - 1);
+    const double2* row = lds + z * R + yb * nt;
+    const double U = Ub ? Ub[l] : 0.0;
+    const double dU = dUb ? dUb[l] : 0.0;
     double u[3], g[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double2 r0 = buf[z * nt + 2 * c];
-      const double2 r1 = buf[z * nt + 2 * c + 1];
+      const double2 r0 = row[2 * c];
+      const double2 r1 = row[2 * c + 1];
       u[c] = r0.x;
       g[c][0] = r0.y;
       g[c][1] = r1.x;
       g[c][2] = r1.y;
     }
     const double div = g[0][0] + g[1][1] + g[2][2];
-    double H[3];
     if (prod == PROD_NS) {
       // H = -(u.grad) u + 0.5 (div u) u   (navier_stokes.py:236-261)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
-        H[c] = -conv + 0.5 * (div * u[c]);
+        Hs[q][c] = -conv + 0.5 * (div * u[c]);
       }
     } else if (prod == PROD_PERT) {
       // H = -(c u.grad u + U d_x u + v U' e_x) + 0.5 (c div u u + div u U e_x)
@@ -264,15 +349,15 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
           hel += u[1] * dU;
           dv += div * U;
         }
-        H[c] = -hel + 0.5 * dv;
+        Hs[q][c] = -hel + 0.5 * dv;
       }
     } else {
-      // dual: v (rows 0..5) adjoint, W (rows 6..11) transport field (without U)
+      // dual: rows 0..5 adjoint v, rows 6..11 the forward field W (plus U)
       double wv[3], gw[3][3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double2 r0 = buf[z * nt + 6 + 2 * c];
-        const double2 r1 = buf[z * nt + 7 + 2 * c];
+        const double2 r0 = row[6 + 2 * c];
+        const double2 r1 = row[7 + 2 * c];
         wv[c] = clin * r0.x;
         gw[c][0] = clin * r0.y;
         gw[c][1] = clin * r1.x;
@@ -287,27 +372,30 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
       for (int c = 0; c < 3; ++c) {
         const double t1 = wv[0] * g[c][0] + wv[1] * g[c][1] + wv[2] * g[c][2];
         const double t2 = u[0] * gw[0][c] + u[1] * gw[1][c] + u[2] * gw[2][c];
-        H[c] = -t1 + t2 + 0.5 * (0.5 * (divw * u[c] + div * wv[c]));
+        Hs[q][c] = -t1 + t2 + 0.5 * (0.5 * (divw * u[c] + div * wv[c]));
       }
     }
-    // packed outputs at natural index z; DIT order applied after the barrier
-    lds[P * nt + P * 2 + z * 2 + 0] = make_double2(H[0], H[1]);
-    lds[P * nt + P * 2 + z * 2 + 1] = make_double2(H[2], 0.0);
   }
   __syncthreads();
-  double2* nat = lds + P * nt + P * 2;
-  for (int w = threadIdx.x; w < P * 2; w += blockDim.x) {
-    const int o = w % 2;
-    const int pos = w / 2;
-    obuf[pos * 2 + o] = nat[PZ.perm[pos] * 2 + o];
+  const int R2 = 2 * YB;
+#pragma unroll
+  for (int q = 0; q < ZMID_ITEMS; ++q) {
+    const int item = threadIdx.x + q * blockDim.x;
+    if (item >= P * YB) continue;
+    const int yb = item / P, z = item - yb * P;
+    const int pos = PZ.iperm[z];
+    lds[pos * R2 + 2 * yb] = make_double2(Hs[q][0], Hs[q][1]);
+    lds[pos * R2 + 2 * yb + 1] = make_double2(Hs[q][2], 0.0);
   }
   __syncthreads();
-  fft_lds(obuf, 2, 2, 1, PZ, -1.0);
-  for (int w = threadIdx.x; w < Z * 2; w += blockDim.x) {
-    const int o = w / Z;
+  fft_lds(lds, R2, R2, 1, PZ, -1.0);
+  for (int w = threadIdx.x; w < Z * R2; w += blockDim.x) {
     const int j = w % Z;
+    const int r = w / Z;
+    const int yb = r / 2, o = r % 2;
+    if (yb >= nyb) continue;
     const int n = (j <= KZ) ? j : P - (Z - j);
-    V[o * vstride + ((size_t)x * N + l) * Z + j] = obuf[n * 2 + o];
+    V[o * vstride + ((size_t)x * N + l0 + yb) * Z + j] = lds[n * R2 + r];
   }
 }
 
@@ -435,11 +523,12 @@ __global__ __launch_bounds__(256) void k_nl_lin(const double2* __restrict__ U0, 
 struct SolveArgs {
   double2* U;  // state, 3 components (in place)
   long cs;
-  const double2* HV;
-  const double2* HG;
+  const double2* H;      // nonlinear term H (3 components, stored convention)
+  double2* HV;           // out: h_v of this substep (the next substep's old term)
+  double2* HG;           // out: h_g
   const double2* HVo;
   const double2* HGo;
-  const double2* CONV;   // (0,0) line of conv_ns: [0..N) comp 0, [N..2N) comp 2
+  double2* CONV;         // out: (0,0) line of conv_ns: [0..N) comp 0, [N..2N) comp 2
   const double2* CONVo;
   const double* D_T;     // N*N transposed
   const double* D2_T;
@@ -447,6 +536,8 @@ struct SolveArgs {
   const double* LI_T;    // lhs_inv[s] transposed, per |k|
   const double* PO_T;    // poisson transposed, per |k|
   const double* VA;      // v_a[s], per |k|
+  const double* DVA;     // D v_a[s], per |k|
+  const double* DVB;     // D flip(v_a)[s], per |k|
   const double* INFL;    // infl_inv[s], per |k|
   const double* RNS;     // rhs_ns[s] (2N)^2 row-major
   const double* LNS;     // lhs_inv_ns[s]
@@ -458,43 +549,82 @@ struct SolveArgs {
   int use_old;
 };
 
-// Y[v][i] = sum_j A[i][j] X[v][j], rows i in [r0, r1); nv vectors
-__device__ void mv_lds(const double* __restrict__ AT, int N, const double* X, double* Y, int nv, int r0, int r1) {
-  const int nr = r1 - r0;
-  const int half = (nv + 1) / 2;
-  for (int it = threadIdx.x; it < 2 * nr; it += blockDim.x) {
-    const int i = r0 + (it % nr);
-    const int h = it / nr;
-    const int v0 = h * half;
-    const int v1 = min(nv, v0 + half);
-    double acc[8];
+// Per-group matvecs on FP64 MFMA.  The group's vectors live in LDS as
+// [y][SLD] (16 columns: column 2p+r is pencil p, re/im r; a second set of 8
+// for omega_y where needed).  Rows N..JSD_NYMAX-1 stay zero so the K padding
+// of the 16x16x4 tiles reads zeros.
+#define SLD 16
+
+// Y[i][c] = sum_j A[i][j] X[j][c] for rows r0 <= i < r1 and columns c < ncol.
+// AT is A transposed (AT[j*N + i]); a wave owns row tiles w, w+4, w+8.  The
+// A fragments of the next MV_KC k-steps are loaded into registers while the
+// current ones feed the MFMAs (the per-|k| tables stream from HBM).
+#define MV_KC 4
+__device__ void mv_mfma(const double* __restrict__ AT, int N, const double* X, double* Y, int ncol, int r0, int r1) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int NT = (N + 15) / 16, KS = (N + 3) / 4;
+  const int kr = lane >> 4, cc = lane & 15;
+  dbl4 acc[3];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) acc[v] = 0.0;
-    for (int j = 0; j < N; ++j) {
-      const double a = AT[(size_t)j * N + i];
+  for (int q = 0; q < 3; ++q) acc[q] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  const int nt = (NT - wave + 3) / 4;  // tiles wave, wave+4, wave+8 below NT
+  int row[3];
 #pragma unroll
-      for (int v = 0; v < 8; ++v)
-        if (v0 + v < v1) acc[v] += a * X[(v0 + v) * JSD_NYMAX + j];
+  for (int q = 0; q < 3; ++q) row[q] = (wave + 4 * q) * 16 + cc;
+  double acur[MV_KC][3], anext[MV_KC][3];
+  auto loadA = [&](int ks0, double (&dst)[MV_KC][3]) {
+#pragma unroll
+    for (int kk = 0; kk < MV_KC; ++kk) {
+      const int k = (ks0 + kk) * 4 + kr;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        dst[kk][q] = (q < nt && k < N && row[q] < N) ? AT[(size_t)k * N + row[q]] : 0.0;
+    }
+  };
+  loadA(0, acur);
+  for (int ks0 = 0; ks0 < KS; ks0 += MV_KC) {
+    if (ks0 + MV_KC < KS) loadA(ks0 + MV_KC, anext);
+#pragma unroll
+    for (int kk = 0; kk < MV_KC; ++kk) {
+      if (ks0 + kk < KS) {
+        const double b = X[((ks0 + kk) * 4 + kr) * SLD + cc];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (q < nt) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(acur[kk][q], b, acc[q], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int v = 0; v < 8; ++v)
-      if (v0 + v < v1) Y[(v0 + v) * JSD_NYMAX + i] = acc[v];
+    for (int kk = 0; kk < MV_KC; ++kk)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acur[kk][q] = anext[kk][q];
+  }
+  if (cc >= ncol) return;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    if (q < nt) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = (wave + 4 * q) * 16 + 4 * r + kr;
+        if (i >= r0 && i < r1) Y[i * SLD + cc] = acc[q][r];
+      }
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
-  __shared__ double sv[8 * JSD_NYMAX];    // v
-  __shared__ double sw[8 * JSD_NYMAX];    // omega_y
-  __shared__ double sphi[16 * JSD_NYMAX]; // [phi | omega] then rhs
-  __shared__ double srhs[16 * JSD_NYMAX];
-  __shared__ double snl[16 * JSD_NYMAX];  // dt*gamma*h + dt*xi*h_old  for [v | omega]
-  __shared__ double sab[4 * 4];           // a, b per pencil (complex)
+  // three [y][16] buffers (rows up to JSD_NYMAX) keep three workgroups per CU
+  __shared__ double X16[JSD_NYMAX * SLD];
+  __shared__ double Y16[JSD_NYMAX * SLD];
+  __shared__ double V16[JSD_NYMAX * SLD];
+  __shared__ double sab[16];
   __shared__ int pis[4], pjs[4];
   __shared__ int npen;
 
   const int nblk = gridDim.x;
-  // XCD-aware order: blocks dispatched round-robin over 8 XCDs; consecutive
-  // logical groups (neighbouring kz) land on the same XCD and share L2 lines.
+  // XCD-aware order: blocks are dispatched round-robin over 8 XCDs;
+  // consecutive logical groups (neighbouring kz) land on the same XCD and
+  // share its L2 lines.
   const int per = (nblk + 7) / 8;
   const int g = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (g >= S.Ka * S.Kb) return;
@@ -508,6 +638,11 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
     if (a > 0 && b > 0) { pis[n] = M - a; pjs[n] = Z - b; ++n; }
     npen = n;
   }
+  for (int it = threadIdx.x; it < JSD_NYMAX * SLD; it += blockDim.x) {
+    X16[it] = 0.0;
+    Y16[it] = 0.0;
+    V16[it] = 0.0;
+  }
   __syncthreads();
   const int np = npen;
   const int nv = 2 * np;
@@ -515,109 +650,121 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
   const double k2 = kxa * kxa + kzb * kzb;
   const size_t kofs = (size_t)g * N * N;
 
-  // ---- load v, omega_y = (i kz) u - (i kx) w, and the explicit terms
+  // ---- load v (V16 cols 0..7), omega_y = (i kz) u - (i kx) w (X16 cols 8..15),
+  //      X = (i kx) H0 + (i kz) H2 (Y16 cols 0..7), H1 (Y16 cols 8..15);
+  //      h_g = (i kz) H0 - (i kx) H2 is final here   (navier_stokes.py:48-73)
   for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
     const int p = it / N, l = it % N;
     const int i = pis[p], j = pjs[p];
     const size_t e = ((size_t)i * N + l) * Z + j;
     const double kxv = S.kx[i], kzv = S.kz[j];
     const double2 u = S.U[e], v = S.U[S.cs + e], w = S.U[2 * S.cs + e];
-    sv[(2 * p) * JSD_NYMAX + l] = v.x;
-    sv[(2 * p + 1) * JSD_NYMAX + l] = v.y;
+    V16[l * SLD + 2 * p] = v.x;
+    V16[l * SLD + 2 * p + 1] = v.y;
     const double2 om = csub(cik(kzv, u), cik(kxv, w));
-    sw[(2 * p) * JSD_NYMAX + l] = om.x;
-    sw[(2 * p + 1) * JSD_NYMAX + l] = om.y;
-    double2 nv_ = cscale(S.HV[e], S.dt_gamma);
-    double2 ng_ = cscale(S.HG[e], S.dt_gamma);
-    if (S.use_old) {
-      nv_ = cadd(nv_, cscale(S.HVo[e], S.dt_xi));
-      ng_ = cadd(ng_, cscale(S.HGo[e], S.dt_xi));
+    X16[l * SLD + 8 + 2 * p] = om.x;
+    X16[l * SLD + 8 + 2 * p + 1] = om.y;
+    const double2 h0 = S.H[e], h1 = S.H[S.cs + e], h2 = S.H[2 * S.cs + e];
+    const double2 xc = cadd(cik(kxv, h0), cik(kzv, h2));
+    Y16[l * SLD + 2 * p] = xc.x;
+    Y16[l * SLD + 2 * p + 1] = xc.y;
+    Y16[l * SLD + 8 + 2 * p] = h1.x;
+    Y16[l * SLD + 8 + 2 * p + 1] = h1.y;
+    S.HG[e] = csub(cik(kzv, h0), cik(kxv, h2));
+    if (i == 0 && j == 0) {
+      S.CONV[l] = make_double2(-h0.x, -h0.y);
+      S.CONV[N + l] = make_double2(-h2.x, -h2.y);
     }
-    snl[(2 * p) * JSD_NYMAX + l] = nv_.x;
-    snl[(2 * p + 1) * JSD_NYMAX + l] = nv_.y;
-    snl[(nv + 2 * p) * JSD_NYMAX + l] = ng_.x;
-    snl[(nv + 2 * p + 1) * JSD_NYMAX + l] = ng_.y;
+  }
+  __syncthreads();
+  // ---- h_v = (-D X + (-kx^2) H1) + (-kz^2) H1
+  mv_mfma(S.D_T, N, Y16, X16, 8, 0, N);
+  __syncthreads();
+  for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
+    const int p = it / N, l = it % N;
+    const int i = pis[p], j = pjs[p];
+    const size_t e = ((size_t)i * N + l) * Z + j;
+    const double kxv = S.kx[i], kzv = S.kz[j];
+    const double2 h1 = make_double2(Y16[l * SLD + 8 + 2 * p], Y16[l * SLD + 8 + 2 * p + 1]);
+    const double2 dx = make_double2(X16[l * SLD + 2 * p], X16[l * SLD + 2 * p + 1]);
+    S.HV[e] = cadd(cadd(cscale(dx, -1.0), cscale(h1, -(kxv * kxv))), cscale(h1, -(kzv * kzv)));
   }
   __syncthreads();
   // ---- phi = ((-kx^2) v + D^2 v) + (-kz^2) v    (navier_stokes.py:1716-1725)
-  mv_lds(S.D2_T, N, sv, srhs, nv, 0, N);
+  mv_mfma(S.D2_T, N, V16, Y16, 8, 0, N);
   __syncthreads();
   for (int it = threadIdx.x; it < nv * N; it += blockDim.x) {
-    const int v = it / N, l = it % N;
-    const int p = v / 2;
+    const int c = it % nv, l = it / nv;
+    const int p = c / 2;
     const double kxv = S.kx[pis[p]], kzv = S.kz[pjs[p]];
-    const double x = sv[v * JSD_NYMAX + l];
-    sphi[v * JSD_NYMAX + l] = ((-(kxv * kxv)) * x + srhs[v * JSD_NYMAX + l]) + (-(kzv * kzv)) * x;
-    sphi[(nv + v) * JSD_NYMAX + l] = sw[v * JSD_NYMAX + l];
+    const double x = V16[l * SLD + c];
+    X16[l * SLD + c] = ((-(kxv * kxv)) * x + Y16[l * SLD + c]) + (-(kzv * kzv)) * x;
   }
   __syncthreads();
-  // ---- rhs = BC(R_s [phi | omega] + explicit)   (navier_stokes.py:1296-1302, :1400-1408)
-  mv_lds(S.RB_T, N, sphi, srhs, 2 * nv, 1, N - 1);
+  // ---- rhs = BC(R_s [phi | omega] + dt gamma h + dt xi h_old)
+  //      (navier_stokes.py:1296-1302, :1400-1408)
+  mv_mfma(S.RB_T, N, X16, Y16, 16, 1, N - 1);
   __syncthreads();
-  for (int it = threadIdx.x; it < 2 * nv * N; it += blockDim.x) {
-    const int v = it / N, l = it % N;
-    double r = 0.0;
-    if (l > 0 && l < N - 1) r = (srhs[v * JSD_NYMAX + l] + S.diag_alpha * (-k2) * sphi[v * JSD_NYMAX + l]) + snl[v * JSD_NYMAX + l];
-    sphi[v * JSD_NYMAX + l] = r;
+  for (int it = threadIdx.x; it < N * 2 * np; it += blockDim.x) {
+    const int l = it / (2 * np), r = it % (2 * np);
+    const int p = r / 2, f = r % 2;  // f = 0: v part (h_v), 1: omega part (h_g)
+    double2 vals[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    if (l > 0 && l < N - 1) {
+      const size_t e = ((size_t)pis[p] * N + l) * Z + pjs[p];
+      double2 h = cscale(f == 0 ? S.HV[e] : S.HG[e], S.dt_gamma);
+      if (S.use_old) h = cadd(h, cscale(f == 0 ? S.HVo[e] : S.HGo[e], S.dt_xi));
+      const int c0 = 8 * f + 2 * p;
+      vals[0] = make_double2((Y16[l * SLD + c0] + S.diag_alpha * (-k2) * X16[l * SLD + c0]) + h.x,
+                             (Y16[l * SLD + c0 + 1] + S.diag_alpha * (-k2) * X16[l * SLD + c0 + 1]) + h.y);
+    }
+    const int c0 = 8 * f + 2 * p;
+    X16[l * SLD + c0] = vals[0].x;
+    X16[l * SLD + c0 + 1] = vals[0].y;
   }
   __syncthreads();
   // ---- [phi' | omega'] = BC(L_s^-1 rhs)
-  mv_lds(S.LI_T + kofs, N, sphi, srhs, 2 * nv, 1, N - 1);
+  mv_mfma(S.LI_T + kofs, N, X16, Y16, 16, 1, N - 1);
   __syncthreads();
-  for (int it = threadIdx.x; it < 2 * nv * 2; it += blockDim.x) {
-    const int v = it / 2;
-    srhs[v * JSD_NYMAX + (it % 2 ? N - 1 : 0)] = 0.0;
-  }
+  if (threadIdx.x < 2 * SLD) Y16[(threadIdx.x < SLD ? 0 : N - 1) * SLD + (threadIdx.x % SLD)] = 0.0;
   __syncthreads();
-  // ---- v_p = BC(P phi')
-  mv_lds(S.PO_T + (size_t)g * N * N, N, srhs, sphi, nv, 1, N - 1);
+  // ---- v_p = BC(P phi')  (X16 columns 0..7; its wall rows are zero)
+  mv_mfma(S.PO_T + kofs, N, Y16, X16, 8, 1, N - 1);
   __syncthreads();
-  for (int it = threadIdx.x; it < nv * 2; it += blockDim.x) {
-    const int v = it / 2;
-    sphi[v * JSD_NYMAX + (it % 2 ? N - 1 : 0)] = 0.0;
-  }
+  // ---- D v_p (V16 columns 0..7)
+  mv_mfma(S.D_T, N, X16, V16, 8, 0, N);
   __syncthreads();
-  // ---- wall correction: [a, b] = Minv (-[D v_p]_0, -[D v_p]_{N-1})
+  // ---- wall correction [a, b] = Minv (-[D v_p]_0, -[D v_p]_{N-1})  (navier_stokes.py:1366-1389)
   if (threadIdx.x < nv) {
-    const int v = threadIdx.x;
-    double d0 = 0.0, d1 = 0.0;
-    for (int j = 0; j < N; ++j) {
-      const double x = sphi[v * JSD_NYMAX + j];
-      d0 += S.D_T[(size_t)j * N + 0] * x;
-      d1 += S.D_T[(size_t)j * N + (N - 1)] * x;
-    }
+    const int c = threadIdx.x;
     const double* mi = S.INFL + (size_t)g * 4;
-    const double r0 = -d0, r1 = -d1;
-    sab[v * 2 + 0] = mi[0] * r0 + mi[1] * r1;
-    sab[v * 2 + 1] = mi[2] * r0 + mi[3] * r1;
+    const double r0 = -V16[c], r1 = -V16[(N - 1) * SLD + c];
+    sab[2 * c + 0] = mi[0] * r0 + mi[1] * r1;
+    sab[2 * c + 1] = mi[2] * r0 + mi[3] * r1;
   }
   __syncthreads();
+  // ---- v = BC(v_p + a v_a + b flip(v_a)); D v = D v_p + a D v_a + b D flip(v_a);
+  //      u, w from continuity (navier_stokes.py:1506-1527)
   const double* va = S.VA + (size_t)g * N;
-  for (int it = threadIdx.x; it < nv * N; it += blockDim.x) {
-    const int v = it / N, l = it % N;
-    double x = 0.0;
-    if (l > 0 && l < N - 1)
-      x = (sphi[v * JSD_NYMAX + l] + sab[v * 2 + 0] * va[l]) + sab[v * 2 + 1] * va[N - 1 - l];
-    sv[v * JSD_NYMAX + l] = x;  // new v (BC applied)
-    sw[v * JSD_NYMAX + l] = (l > 0 && l < N - 1) ? srhs[(nv + v) * JSD_NYMAX + l] : 0.0;  // new omega
-  }
-  __syncthreads();
-  // ---- D v_new
-  mv_lds(S.D_T, N, sv, srhs, nv, 0, N);
-  __syncthreads();
-  // ---- write u, v, w
+  const double* dva = S.DVA + (size_t)g * N;
+  const double* dvb = S.DVB + (size_t)g * N;
   for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
     const int p = it / N, l = it % N;
     const int i = pis[p], j = pjs[p];
     const size_t e = ((size_t)i * N + l) * Z + j;
     const bool wall = (l == 0 || l == N - 1);
-    const double2 vnew = make_double2(sv[(2 * p) * JSD_NYMAX + l], sv[(2 * p + 1) * JSD_NYMAX + l]);
+    const int cr = 2 * p, ci = 2 * p + 1;
+    double2 vnew = make_double2(0.0, 0.0);
+    if (!wall) {
+      vnew.x = (X16[l * SLD + cr] + sab[2 * cr] * va[l]) + sab[2 * cr + 1] * va[N - 1 - l];
+      vnew.y = (X16[l * SLD + ci] + sab[2 * ci] * va[l]) + sab[2 * ci + 1] * va[N - 1 - l];
+    }
     S.U[S.cs + e] = vnew;
     if (i == 0 && j == 0) continue;  // mean flow below
     const double kxv = S.kx[i], kzv = S.kz[j];
     const double mk2 = -(kxv * kxv + kzv * kzv);
-    const double dr = srhs[(2 * p) * JSD_NYMAX + l], di = srhs[(2 * p + 1) * JSD_NYMAX + l];
-    const double orr = sw[(2 * p) * JSD_NYMAX + l], oi = sw[(2 * p + 1) * JSD_NYMAX + l];
+    const double dr = (V16[l * SLD + cr] + sab[2 * cr] * dva[l]) + sab[2 * cr + 1] * dvb[l];
+    const double di = (V16[l * SLD + ci] + sab[2 * ci] * dva[l]) + sab[2 * ci + 1] * dvb[l];
+    const double orr = wall ? 0.0 : Y16[l * SLD + 8 + cr], oi = wall ? 0.0 : Y16[l * SLD + 8 + ci];
     // u = (-(i kx) Dv + (i kz) om) / (-k^2); w = (-(i kz) Dv - (i kx) om) / (-k^2)
     double2 u = make_double2((kxv * di + (-kzv * oi)) / mk2, (-kxv * dr + kzv * orr) / mk2);
     double2 w = make_double2((kzv * di - (-kxv * oi)) / mk2, (-kzv * dr - kxv * orr) / mk2);
@@ -631,8 +778,8 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
   if (a != 0 || b != 0) return;
   __syncthreads();
   // ---- (0,0) mean flow: [u;w] = Lns^-1 (Rns [u;w] + dt g N_new + dt x N_old)  (navier_stokes.py:1417-1504)
-  double* x2 = snl;   // 2N complex as [re 2N][im 2N]
-  double* y2 = sphi;
+  double* x2 = V16;  // 2N complex as [re 2N][im 2N]
+  double* y2 = Y16;
   const int N2 = 2 * N;
   for (int l = threadIdx.x; l < N2; l += blockDim.x) {
     const size_t e = ((size_t)(l % N)) * Z;  // (i=0, l, j=0)
@@ -680,6 +827,8 @@ __global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
     else S.U[2 * S.cs + e] = val;
   }
 }
+
+#include "jsd_adjoint.inc"
 
 // ===========================================================================
 // energy: E = sum_c sum_l (w_l / 4) s^2/(Px Pz) sum_k |Herm(u_c)|^2
@@ -754,8 +903,9 @@ struct jsd_plan {
   double s_inv, s_fwd, dpdx_scale;
   std::vector<void*> allocs;
   // device tables
-  double *DF1, *DF2, *D_T, *D2_T, *RB_T, *LI_T, *PO_T, *VA, *INFL, *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
-  int *permx, *permz;
+  double *DF1, *DF2, *DFT1, *D_R, *D2_R, *RB_R, *LI_R, *PO_R;  // adjoint: D^T tiles, row-major tables
+  double *D_T, *D2_T, *RB_T, *LI_T, *PO_T, *VA, *DVA, *DVB, *INFL, *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
+  int *permx, *permz, *ipermx, *ipermz;
   double2 *twx, *twz;
   FftPlan fpx, fpz;
   // workspaces
@@ -837,7 +987,7 @@ static void dit_order(int len, const std::vector<int>& rs, int nrs, std::vector<
     for (int n1 : sub) out.push_back(r * n1 + n2);
 }
 
-static int make_fft(jsd_plan* p, int len, FftPlan* fp, int** perm, double2** tw) {
+static int make_fft(jsd_plan* p, int len, FftPlan* fp, int** perm, int** iperm, double2** tw) {
   std::vector<int> rad;
   if (!factor_plan(len, rad)) JSD_FAIL(JSD_ERR_ARG, "transform length %d is not a product of 2,3,4,5,7,8", len);
   std::vector<int> order;
@@ -847,7 +997,11 @@ static int make_fft(jsd_plan* p, int len, FftPlan* fp, int** perm, double2** tw)
     const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)len;
     t[m] = make_double2((double)cosl(ang), -(double)sinl(ang));
   }
+  std::vector<int> inv(len);
+  for (int q = 0; q < len; ++q) inv[order[q]] = q;
   int rc = upload(p, perm, order.data(), len);
+  if (rc) return rc;
+  rc = upload(p, iperm, inv.data(), len);
   if (rc) return rc;
   rc = upload(p, tw, t.data(), len);
   if (rc) return rc;
@@ -855,6 +1009,7 @@ static int make_fft(jsd_plan* p, int len, FftPlan* fp, int** perm, double2** tw)
   fp->nrad = (int)rad.size();
   for (int s = 0; s < fp->nrad; ++s) fp->rad[s] = rad[s];
   fp->perm = *perm;
+  fp->iperm = *iperm;
   fp->tw = *tw;
   return JSD_OK;
 }
@@ -947,7 +1102,39 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     TRY(upload(p, &p->LI_T, li.data(), li.size()));
     TRY(upload(p, &p->PO_T, po.data(), po.size()));
   }
+  p->DFT1 = p->D_R = p->D2_R = p->RB_R = p->LI_R = p->PO_R = nullptr;
+  if (d->kind == JSD_KIND_PERTURBATION && d->linearise) {
+    // the discrete adjoint applies the transposes: row-major copies of the
+    // tables (mv_mfma reads its operand transposed) and D^T fragments
+    std::vector<double> DT(nn);
+    transpose_sq(d->D, DT.data(), N);
+    tile_D(DT.data(), N, p->NT, p->KS, tiled);
+    TRY(upload(p, &p->DFT1, tiled.data(), tiled.size()));
+    TRY(upload(p, &p->D_R, d->D, nn));
+    TRY(upload(p, &p->D2_R, D2.data(), nn));
+    TRY(upload(p, &p->RB_R, d->rhs_base, 3 * nn));
+    TRY(upload(p, &p->LI_R, d->lhs_inv, 3 * nk * nn));
+    TRY(upload(p, &p->PO_R, d->poisson, nk * nn));
+  }
   TRY(upload(p, &p->VA, d->v_a, 3 * nk * N));
+  {
+    // D v_a and D flip(v_a) per (substep, |k|) for the wall-corrected derivative
+    std::vector<double> dva(3 * nk * N), dvb(3 * nk * N);
+    for (size_t k = 0; k < 3 * nk; ++k) {
+      const double* va = d->v_a + k * N;
+      for (int r = 0; r < N; ++r) {
+        double sa = 0.0, sb = 0.0;
+        for (int c = 0; c < N; ++c) {
+          sa += d->D[(size_t)r * N + c] * va[c];
+          sb += d->D[(size_t)r * N + c] * va[N - 1 - c];
+        }
+        dva[k * N + r] = sa;
+        dvb[k * N + r] = sb;
+      }
+    }
+    TRY(upload(p, &p->DVA, dva.data(), dva.size()));
+    TRY(upload(p, &p->DVB, dvb.data(), dvb.size()));
+  }
   TRY(upload(p, &p->INFL, d->infl_inv, 3 * nk * 4));
   TRY(upload(p, &p->RNS, d->rhs_ns, 3 * 4 * nn));
   TRY(upload(p, &p->LNS, d->lhs_inv_ns, 3 * 4 * nn));
@@ -960,8 +1147,8 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   } else {
     p->Ub = p->dUb = nullptr;
   }
-  TRY(make_fft(p, d->ppx, &p->fpx, &p->permx, &p->twx));
-  TRY(make_fft(p, d->ppz, &p->fpz, &p->permz, &p->twz));
+  TRY(make_fft(p, d->ppx, &p->fpx, &p->permx, &p->ipermx, &p->twx));
+  TRY(make_fft(p, d->ppz, &p->fpz, &p->permz, &p->ipermz, &p->twz));
   const size_t fb = p->fs * sizeof(double2);
   TRY(dalloc(p, (void**)&p->DU, 3 * fb));
   TRY(dalloc(p, (void**)&p->H, 3 * fb));
@@ -999,7 +1186,7 @@ static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int 
   dim3 grid(p->M, (2 * p->Z + 63) / 64, ncomp);
   prof_begin(p, 0, st);
   hipLaunchKernelGGL(k_ygemm<GEMM_PLAIN>, grid, dim3(256), 0, st, order == 2 ? p->DF2 : p->DF1, p->NT, p->KS,
-                     (const double*)in, nullptr, nullptr, (double*)out, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
+                     (const double*)in, nullptr, nullptr, (double*)out, nullptr, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
                      p->Z, (long)(2 * p->fs));
   prof_end(p, 0, st);
   HIPCHK(hipGetLastError());
@@ -1011,7 +1198,7 @@ static int launch_hv(jsd_plan* p, const double2* H, double2* hv, double2* hg, do
   const double* h = (const double*)H;
   prof_begin(p, 4, st);
   hipLaunchKernelGGL(k_ygemm<GEMM_HV>, grid, dim3(256), 0, st, p->DF1, p->NT, p->KS, h, h + 2 * p->fs,
-                     h + 4 * p->fs, (double*)hv, (double*)hg, (double*)conv, p->kx, p->kz, p->M, p->N, p->Z, 0L);
+                     h + 4 * p->fs, (double*)hv, (double*)hg, nullptr, (double*)conv, p->kx, p->kz, p->M, p->N, p->Z, 0L);
   prof_end(p, 4, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
@@ -1039,17 +1226,9 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   const int nqb = (p->KZ + 1 + QB - 1) / QB;
   const size_t ts = (size_t)d.ppx * N * Z;
   const size_t lds_x = (size_t)d.ppx * NSLOT * sizeof(double2);
-  int ntr = 6;
-  prof_begin(p, 1, st);
-  for (int c = 0; c < 3; ++c) {
-    hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, U + c * fs, (int)OP_ID, U + c * fs,
-                       (int)OP_DX, p->T + (2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
-    hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, p->DU + c * fs, (int)OP_ID, U + c * fs,
-                       (int)OP_DZ, p->T + (2 * c + 1) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
-  }
-  prof_end(p, 1, st);
   int prod = PROD_NS;
   double clin = 1.0;
+  int ntr = 6;
   if (d.kind == JSD_KIND_PERTURBATION) prod = PROD_PERT;
   if (d.kind == JSD_KIND_DUAL) {
     prod = PROD_DUAL;
@@ -1057,19 +1236,42 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     if (!UF) JSD_FAIL(JSD_ERR_ARG, "nonlinear dual step needs the forward state");
     rc = launch_ygemm_plain(p, UF, p->DWF, 3, 1, st);
     if (rc) return rc;
-    for (int c = 0; c < 3; ++c) {
-      hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, UF + c * fs, (int)OP_ID, UF + c * fs,
-                         (int)OP_DX, p->T + (6 + 2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z, p->s_inv);
-      hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N), dim3(256), lds_x, st, p->DWF + c * fs, (int)OP_ID,
-                         UF + c * fs, (int)OP_DZ, p->T + (7 + 2 * c) * ts, p->fpx, p->kx, p->kz, M, N, Z,
-                         p->s_inv);
+  }
+  XInvSet set;
+  for (int c = 0; c < 3; ++c) {
+    // (u_c, d_x u_c) and (d_y u_c, d_z u_c)
+    set.A[2 * c] = U + c * fs;
+    set.opA[2 * c] = OP_ID;
+    set.B[2 * c] = U + c * fs;
+    set.opB[2 * c] = OP_DX;
+    set.A[2 * c + 1] = p->DU + c * fs;
+    set.opA[2 * c + 1] = OP_ID;
+    set.B[2 * c + 1] = U + c * fs;
+    set.opB[2 * c + 1] = OP_DZ;
+    if (ntr == 12) {
+      set.A[6 + 2 * c] = UF + c * fs;
+      set.opA[6 + 2 * c] = OP_ID;
+      set.B[6 + 2 * c] = UF + c * fs;
+      set.opB[6 + 2 * c] = OP_DX;
+      set.A[7 + 2 * c] = p->DWF + c * fs;
+      set.opA[7 + 2 * c] = OP_ID;
+      set.B[7 + 2 * c] = UF + c * fs;
+      set.opB[7 + 2 * c] = OP_DZ;
     }
   }
+  for (int t = 0; t < ntr; ++t) set.T[t] = p->T + t * ts;
+  prof_begin(p, 1, st);
+  hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N, ntr), dim3(256), lds_x, st, set, p->fpx, p->kx, p->kz, M, N, Z,
+                     p->s_inv);
+  prof_end(p, 1, st);
   HIPCHK(hipGetLastError());
-  const size_t lds_z = ((size_t)d.ppz * ntr + (size_t)d.ppz * 4) * sizeof(double2);
+  // y planes per z-pass workgroup: LDS <= 80 KB and <= ZMID_ITEMS items per thread
+  int YB = (int)((80 * 1024) / ((size_t)d.ppz * ntr * sizeof(double2)));
+  YB = std::max(1, std::min(YB, std::min(4, (ZMID_ITEMS * 256) / d.ppz)));
+  const size_t lds_z = (size_t)d.ppz * ntr * YB * sizeof(double2);
   prof_begin(p, 2, st);
-  hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, N), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V, (long)ts, p->fpz,
-                     N, Z, prod, clin, p->Ub, p->dUb);
+  hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, (N + YB - 1) / YB), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V,
+                     (long)ts, p->fpz, N, Z, prod, clin, p->Ub, p->dUb, YB);
   prof_end(p, 2, st);
   HIPCHK(hipGetLastError());
   prof_begin(p, 3, st);
@@ -1086,13 +1288,12 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   static const double alp[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
   int rc = nonlinear_H(p, U, UF, st);
   if (rc) return rc;
-  rc = launch_hv(p, p->H, p->HV, p->HG, p->CONV, st);
-  if (rc) return rc;
   const size_t nn = (size_t)p->N * p->N;
   const size_t nk = (size_t)p->Ka * p->Kb;
   SolveArgs S;
   S.U = U;
   S.cs = (long)p->fs;
+  S.H = p->H;
   S.HV = p->HV;
   S.HG = p->HG;
   S.HVo = p->HVo;
@@ -1105,6 +1306,8 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   S.LI_T = p->LI_T + (size_t)s * nk * nn;
   S.PO_T = p->PO_T;
   S.VA = p->VA + (size_t)s * nk * p->N;
+  S.DVA = p->DVA + (size_t)s * nk * p->N;
+  S.DVB = p->DVB + (size_t)s * nk * p->N;
   S.INFL = p->INFL + (size_t)s * nk * 4;
   S.RNS = p->RNS + (size_t)s * 4 * nn;
   S.LNS = p->LNS + (size_t)s * 4 * nn;
@@ -1215,5 +1418,96 @@ extern "C" int jsd_profile(jsd_plan* p, int enable, double* ms, long* count) {
   }
   for (int c = 0; c < JSD_NPROF; ++c) p->prof_used[c] = 0;
   p->prof_on = enable ? 1 : 0;
+  return JSD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// discrete adjoint of the linearised perturbation step
+// ---------------------------------------------------------------------------
+static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
+  static const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  static const double xi[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+  static const double alp[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
+  const size_t nn = (size_t)p->N * p->N;
+  const size_t nk = (size_t)p->Ka * p->Kb;
+  SolveTArgs S;
+  S.UB = UB;
+  S.cs = (long)p->fs;
+  S.HVB = p->HV;
+  S.HGB = p->HG;
+  S.HVO = p->HVo;
+  S.HGO = p->HGo;
+  S.CB = p->CONV;
+  S.CBO = p->CONVo;
+  S.D_R = p->D_R;
+  S.D2_R = p->D2_R;
+  S.RB_R = p->RB_R + s * nn;
+  S.LI_R = p->LI_R + (size_t)s * nk * nn;
+  S.PO_R = p->PO_R;
+  S.VA = p->VA + (size_t)s * nk * p->N;
+  S.INFL = p->INFL + (size_t)s * nk * 4;
+  S.RNS = p->RNS + (size_t)s * 4 * nn;
+  S.LNS = p->LNS + (size_t)s * 4 * nn;
+  S.kx = p->kx;
+  S.kz = p->kz;
+  S.M = p->M;
+  S.N = p->N;
+  S.Z = p->Z;
+  S.Ka = p->Ka;
+  S.Kb = p->Kb;
+  const double dt = p->d.dt;
+  S.dt_gamma = dt * gam[s];
+  S.dt_xi = dt * xi[s];
+  S.diag_alpha = alp[s] * fabs(dt) / fabs(p->d.re_tau);
+  S.use_old = s > 0;
+  S.has_carry = has_carry ? 1 : 0;
+  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, S);
+  HIPCHK(hipGetLastError());
+  // Hbar from (hvbar, hgbar, convbar): transposed h_v / h_g assembly
+  dim3 grid(p->M, (2 * p->Z + 63) / 64, 1);
+  double* hb = (double*)p->H;
+  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, grid, dim3(256), 0, st, p->DFT1, p->NT, p->KS, (const double*)p->HV,
+                     (const double*)p->HG, nullptr, hb, hb + 2 * p->fs, hb + 4 * p->fs, (double*)p->CONV, p->kx,
+                     p->kz, p->M, p->N, p->Z, 0L);
+  HIPCHK(hipGetLastError());
+  // state adjoint += transposed linearised product term; Dvbar -> DU[0]
+  const size_t tot = p->fs;
+  hipLaunchKernelGGL(k_nl_lin_T, dim3((tot + 255) / 256), dim3(256), 0, st, p->H, (long)p->fs, UB, p->DU, p->kx,
+                     p->kz, p->Ub, p->dUb, p->M, p->N, p->Z);
+  HIPCHK(hipGetLastError());
+  // vbar += D^T Dvbar
+  hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, grid, dim3(256), 0, st, p->DFT1, p->NT, p->KS, (const double*)p->DU,
+                     nullptr, nullptr, (double*)(UB + p->fs), nullptr, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
+                     p->Z, 0L);
+  HIPCHK(hipGetLastError());
+  // carries of this substep's old-term usage feed the previous substep
+  std::swap(p->HV, p->HVo);
+  std::swap(p->HG, p->HGo);
+  std::swap(p->CONV, p->CONVo);
+  return JSD_OK;
+}
+
+extern "C" int jsd_step_vjp(jsd_plan* p, void* grad, int nsteps, void* stream) {
+  if (!p || !grad || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (p->d.kind != JSD_KIND_PERTURBATION || !p->d.linearise || !p->LI_R)
+    JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint is built for linearised perturbation plans");
+  hipStream_t st = (hipStream_t)stream;
+  for (int n = 0; n < nsteps; ++n)
+    for (int s = 2; s >= 0; --s) {
+      int rc = adjoint_substep(p, (double2*)grad, s, s < 2, st);
+      if (rc) return rc;
+    }
+  return JSD_OK;
+}
+
+extern "C" int jsd_energy_grad(jsd_plan* p, const void* vel_hat, void* out, double scale, void* stream) {
+  if (!p || !vel_hat || !out) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  const double sl = (p->d.lx / (2 * M_PI)) * (p->d.lz / (2 * M_PI));
+  const double c = scale * sl * sl / (2.0 * (double)p->d.px * (double)p->d.pz);
+  const size_t tot = 3 * p->fs;
+  hipLaunchKernelGGL(k_energy_grad, dim3((tot + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     (const double2*)vel_hat, (long)p->fs, (double2*)out, p->wy, p->M, p->N, p->Z, c);
+  HIPCHK(hipGetLastError());
   return JSD_OK;
 }

@@ -9,8 +9,8 @@
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-#define JSD_NTMAX 9      // 16-row MFMA tiles along y: ny <= 144
-#define JSD_NYMAX 144
+#define JSD_NTMAX 9      // 16-row MFMA tiles along y
+#define JSD_NYMAX 132     // Chebyshev points (and 4-row K padding) per pencil
 #define JSD_MAXRAD 8     // FFT stages
 
 struct FftPlan {
@@ -18,6 +18,7 @@ struct FftPlan {
   int nrad;            // number of stages
   int rad[JSD_MAXRAD]; // radices, innermost first
   const int* perm;     // LDS position -> input index (device)
+  const int* iperm;    // input index -> LDS position (device)
   const double2* tw;   // forward twiddles exp(-2 pi i m / p) (device)
 };
 

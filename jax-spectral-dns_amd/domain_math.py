@@ -61,15 +61,25 @@ def factorize(n: int) -> List[int]:
     return out if rest == 1 else []
 
 
-def transform_size(m: int, reference_p: int) -> int:
-    """Smallest P' >= 3K+1 (K = m // 2) made of the kernel's radices.  Any such
-    grid computes the same |k| <= K spectrum of a product of two fields with
-    |k| <= K as the reference's grid, up to rounding."""
+def transform_size(m: int, reference_p: int, direction: str = "x") -> int:
+    """Transform grid P' >= 3K+1 (K = m // 2).  Any such grid computes the same
+    |k| <= K spectrum of a product of two fields with |k| <= K as the
+    reference's grid, up to rounding.  In x the intermediate fields have P'
+    rows in HBM, so the smallest size with cheap radices (2, 3, 4, 5, 8) is
+    taken; the z transform lives entirely in LDS, so a power of two (the
+    cheapest butterflies) is taken there."""
     k = m // 2
     p = max(3 * k + 1, 2)
-    while not factorize(p):
+    if direction == "z":
+        q = 1
+        while q < p:
+            q *= 2
+        return q
+    while True:
+        f = factorize(p)
+        if f and 7 not in f:
+            return p
         p += 1
-    return p
 
 
 def cheb_grid(n: int) -> np.ndarray:
@@ -147,7 +157,7 @@ def make_grid(
         # when m = 3 (mod 4); that case is inconsistent upstream
         raise ValueError("stored mode counts must be 1 mod 4 (got %d, %d)" % (nxm, nzm))
     if transform_grid is None:
-        ppx, ppz = transform_size(nxm, px), transform_size(nzm, pz)
+        ppx, ppz = transform_size(nxm, px, "x"), transform_size(nzm, pz, "z")
     else:
         ppx, ppz = transform_grid
     y = cheb_grid(ny)

@@ -95,6 +95,17 @@ class StepPlan:
         self._keep = []  # tables were copied to the device
 
     @property
+    def adjoint_needs_state(self) -> bool:
+        """The linearised step's transpose is independent of the state."""
+        return not (self.kind == _lib.KIND_PERTURBATION and self.linearise)
+
+    def vjp(self, grad: torch.Tensor, nsteps: int = 1) -> torch.Tensor:
+        """In place: adjoint after nsteps steps -> adjoint before them."""
+        self._check_state(grad)
+        _lib.check(_lib.load().jsd_step_vjp(self._h, _dev(grad), int(nsteps), _stream()))
+        return grad
+
+    @property
     def shape(self) -> Tuple[int, int, int, int]:
         return (3, self.grid.nxm, self.grid.ny, self.grid.nzm)
 

@@ -1,0 +1,96 @@
+"""Discrete adjoint of the linearised perturbation step behind
+``torch.autograd`` (the build's counterpart of reverse-mode AD through the
+reference's scan, jax.value_and_grad in optimiser.py:115-121).
+
+``run(u0, plan, nsteps, checkpoint_every)`` advances a complex128 state
+(given as its float64 real-pair view, shape (3, Nx, Ny, Nz, 2)) through
+nsteps RK3/CN steps on the GPU; its backward applies the hand-written
+transposed kernels (``jsd_step_vjp``).  The forward keeps a checkpoint every
+``checkpoint_every`` steps and the backward walks the chunks in reverse,
+recomputing a chunk's states only when the adjoint needs them: the
+linearised step is linear, so its transpose needs no forward state and no
+recompute happens.  ``energy(u, plan)`` is the energy functional with its
+gradient from ``jsd_energy_grad``; ``gain_and_grad`` evaluates
+J = E(u_T)/E(u_0) and dJ/du_0.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Tuple
+
+import torch
+
+from . import _lib
+from .plan import StepPlan, _dev, _stream
+
+
+def _vjp(plan: StepPlan, g: torch.Tensor, nsteps: int) -> None:
+    _lib.check(_lib.load().jsd_step_vjp(plan._h, _dev(g), int(nsteps), _stream()))
+
+
+def _energy_grad(plan: StepPlan, u: torch.Tensor, scale: float) -> torch.Tensor:
+    out = torch.empty_like(u)
+    _lib.check(_lib.load().jsd_energy_grad(plan._h, _dev(u), _dev(out), float(scale), _stream()))
+    return out
+
+
+class _Run(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int):  # type: ignore[override]
+        u = torch.view_as_complex(u0r.contiguous()).clone()
+        k = max(1, min(checkpoint_every, max(1, nsteps)))
+        chunks = []
+        done = 0
+        while done < nsteps:
+            c = min(k, nsteps - done)
+            chunks.append((done, c, u.clone() if plan.adjoint_needs_state else None))
+            plan.step(u, 0.0, c)
+            done += c
+        ctx.plan = plan
+        ctx.chunks = chunks
+        return torch.view_as_real(u).clone()
+
+    @staticmethod
+    def backward(ctx, gr: torch.Tensor):  # type: ignore[override]
+        plan: StepPlan = ctx.plan
+        g = torch.view_as_complex(gr.contiguous()).clone()
+        for start, count, state in reversed(ctx.chunks):
+            # a nonlinear adjoint would recompute the chunk's states from `state` here
+            _vjp(plan, g, count)
+        return torch.view_as_real(g), None, None, None
+
+
+class _Energy(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ur: torch.Tensor, plan: StepPlan):  # type: ignore[override]
+        u = torch.view_as_complex(ur.contiguous())
+        ctx.plan = plan
+        ctx.save_for_backward(ur)
+        return torch.tensor(plan.energy(u), dtype=torch.float64, device=ur.device)
+
+    @staticmethod
+    def backward(ctx, ge: torch.Tensor):  # type: ignore[override]
+        (ur,) = ctx.saved_tensors
+        u = torch.view_as_complex(ur.contiguous())
+        g = _energy_grad(ctx.plan, u, float(ge))
+        return torch.view_as_real(g), None
+
+
+def run(u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int = 50) -> torch.Tensor:
+    if plan.kind != _lib.KIND_PERTURBATION or not plan.linearise:
+        raise _lib.JsdError("the discrete adjoint is built for linearised perturbation plans")
+    return _Run.apply(u0r, plan, nsteps, checkpoint_every)
+
+
+def energy(ur: torch.Tensor, plan: StepPlan) -> torch.Tensor:
+    return _Energy.apply(ur, plan)
+
+
+def gain_and_grad(u0: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int = 50) -> Tuple[float, torch.Tensor]:
+    """J = E(u_T) / E(u_0) and dJ/du_0 (complex128, real-pair convention:
+    d/dRe + i d/dIm)."""
+    x = torch.view_as_real(u0.contiguous()).clone().requires_grad_(True)
+    J = energy(run(x, plan, nsteps, checkpoint_every), plan) / energy(x, plan)
+    (gx,) = torch.autograd.grad(J, x)
+    return float(J), torch.view_as_complex(gx.contiguous())

@@ -646,10 +646,12 @@ def solve_wavenumbers(
     j_kx = 1j * kx
     j_kz = 1j * kz
     minus_k2 = -(kx**2 + kz**2)
+    # the (0,0) entry is replaced by the mean-flow solve below; a unit
+    # denominator there keeps the value unused and the derivative finite
+    minus_k2 = torch.where(minus_k2 == 0, torch.ones_like(minus_k2), minus_k2)
     v_y = Dv(v_new)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        u_new = (-j_kx * v_y + j_kz * vort_new) / minus_k2
-        w_new = (-j_kz * v_y - j_kx * vort_new) / minus_k2
+    u_new = (-j_kx * v_y + j_kz * vort_new) / minus_k2
+    w_new = (-j_kz * v_y - j_kx * vort_new) / minus_k2
     # (0,0): mean-flow block solve (navier_stokes.py:1417-1504)
     n = dom.ny
     dpdx = sp.dPdx * ((dom.px * (2 * np.pi / dom.lx) ** 2) * (dom.pz * (2 * np.pi / dom.lz) ** 2)) ** 0.5
