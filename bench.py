@@ -69,6 +69,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument(
+        "--copy-probe",
+        action="store_true",
+        help="run one 256 MiB device-to-device elementwise copy (torch.neg) first: the HBM-counter "
+        "calibration kernel for profiler runs",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -101,6 +107,12 @@ def main() -> int:
     u0_t = torch.from_numpy(u0)
     u = u0_t.to(dev)
     lib = _lib.load()
+    if args.copy_probe:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     # warm-up, with every kernel class timed to find the dominant one
     lib.jsd_profile(plan._h, 1, None, None)

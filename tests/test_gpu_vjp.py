@@ -4,9 +4,10 @@
   and the GPU adjoint (real-pair inner product), independent of the oracle;
 * the gain gradient dJ/du0, J = E(u_T)/E(u_0), against torch.autograd of
   the oracle (the stand-in for jax.grad of the reference), <= 1e-9;
-* the same gradient against centred finite differences (h = 1e-6) along 16
-  seeded random directions, <= 1e-7 relative (the FD error bound itself is
-  ~1e-10 for these scales).
+* the same gradient against centred finite differences along 16 seeded
+  random directions, <= 1e-8 relative: numerator and denominator energies
+  are exact quadratics along a line, so their centred differences carry no
+  truncation error and only ~1e-14 rounding.
 """
 
 import numpy as np
@@ -75,12 +76,18 @@ def test_gain_gradient_against_finite_differences():
     plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=True)
     J, grad = gain_and_grad(u0, plan, n, checkpoint_every=50)
 
-    def gain(u):
+    def energies(u):
         v = u.clone()
         plan.step(v, 0.0, n)
-        return plan.energy(v) / plan.energy(u)
+        return plan.energy(v), plan.energy(u)
 
-    h = 1e-6
+    # The step is linear and the energy quadratic, so N(e) = E(A(u0 + e d)) and
+    # D(e) = E(u0 + e d) are quadratics in e: their centred differences are
+    # exact for any h (no truncation error), and h of the size of u0 keeps the
+    # rounding noise at ~1e-14 relative.  J' = (N' D - N D') / D^2.
+    h = 0.5
+    N0, D0 = energies(u0)
+    assert abs(N0 / D0 - J) <= 1e-12 * abs(J)
     worst = 0.0
     for k in range(16):
         rng = np.random.default_rng(1 + k)
@@ -88,9 +95,12 @@ def test_gain_gradient_against_finite_differences():
         dirn[:, :, 0] = 0
         dirn[:, :, -1] = 0
         dirn *= torch.linalg.norm(u0) / torch.linalg.norm(dirn)
-        fd = (gain(u0 + h * dirn) - gain(u0 - h * dirn)) / (2 * h)
+        Np, Dp = energies(u0 + h * dirn)
+        Nm, Dm = energies(u0 - h * dirn)
+        dN, dD = (Np - Nm) / (2 * h), (Dp - Dm) / (2 * h)
+        fd = (dN * D0 - N0 * dD) / (D0 * D0)
         ad = _dot(grad, dirn)
         e = abs(fd - ad) / max(abs(ad), 1e-300)
         worst = max(worst, e)
         print("dir %2d fd %.12e ad %.12e rel %.2e" % (k, fd, ad, e))
-    assert worst < 1e-7
+    assert worst < 1e-8
