@@ -19,10 +19,12 @@
 // Reference: jax_spectral_dns navier_stokes.py:1219-1775 (step),
 // navier_stokes.py:48-261 and navier_stokes_perturbation*.py (NL terms),
 // domain.py:716-764 / 1167-1242 (transforms).
+#include "jsd_fft_ct.h"
 #include "jsd_kernels.h"
 
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -276,6 +278,69 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(XInvSet set, FftPlan PX, cons
 // Physical-space product variants
 enum { PROD_NS = 0, PROD_PERT = 1, PROD_DUAL = 2 };
 
+// Skew-symmetric products at one physical point.  row[2c] = (u_c, d_x u_c),
+// row[2c+1] = (d_y u_c, d_z u_c); the dual adds the forward field in rows
+// 6..11 (the base flow U(y) is added here).  H gets the three components.
+template <int PROD>
+__device__ __forceinline__ void zmid_products(const double2* row, double U, double dU, double clin, double* H) {
+  double u[3], g[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 r0 = row[2 * c];
+    const double2 r1 = row[2 * c + 1];
+    u[c] = r0.x;
+    g[c][0] = r0.y;
+    g[c][1] = r1.x;
+    g[c][2] = r1.y;
+  }
+  const double div = g[0][0] + g[1][1] + g[2][2];
+  if constexpr (PROD == PROD_NS) {
+    // H = -(u.grad) u + 0.5 (div u) u   (navier_stokes.py:236-261)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
+      H[c] = -conv + 0.5 * (div * u[c]);
+    }
+  } else if constexpr (PROD == PROD_PERT) {
+    // H = -(c u.grad u + U d_x u + v U' e_x) + 0.5 (c div u u + div u U e_x)
+    // (navier_stokes_perturbation.py:160-239)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
+      double hel = clin * conv + U * g[c][0];
+      double dv = clin * (div * u[c]);
+      if (c == 0) {
+        hel += u[1] * dU;
+        dv += div * U;
+      }
+      H[c] = -hel + 0.5 * dv;
+    }
+  } else {
+    // dual: rows 0..5 adjoint v, rows 6..11 the forward field W (plus U)
+    double wv[3], gw[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2 r0 = row[6 + 2 * c];
+      const double2 r1 = row[7 + 2 * c];
+      wv[c] = clin * r0.x;
+      gw[c][0] = clin * r0.y;
+      gw[c][1] = clin * r1.x;
+      gw[c][2] = clin * r1.y;
+    }
+    wv[0] += U;
+    gw[0][1] += dU;
+    const double divw = gw[0][0] + gw[1][1] + gw[2][2];
+    // H = -(W.grad) v + [sum_j v_j d_i W_j] + 0.25 ((div W) v + (div v) W)
+    // (navier_stokes_perturbation_dual.py:214-294)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double t1 = wv[0] * g[c][0] + wv[1] * g[c][1] + wv[2] * g[c][2];
+      const double t2 = u[0] * gw[0][c] + u[1] * gw[1][c] + u[2] * gw[2][c];
+      H[c] = -t1 + t2 + 0.5 * (0.5 * (divw * u[c] + div * wv[c]));
+    }
+  }
+}
+
 // Middle z pass: inverse z-FFT of nt packed inputs, products, forward z-FFT of
 // two packed outputs, for YB consecutive y planes of one x.  grid
 // (PXlen, ceil(N/YB)), block 256.  LDS rows r = yb*nt + t hold element
@@ -319,62 +384,9 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
     const double2* row = lds + z * R + yb * nt;
     const double U = Ub ? Ub[l] : 0.0;
     const double dU = dUb ? dUb[l] : 0.0;
-    double u[3], g[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double2 r0 = row[2 * c];
-      const double2 r1 = row[2 * c + 1];
-      u[c] = r0.x;
-      g[c][0] = r0.y;
-      g[c][1] = r1.x;
-      g[c][2] = r1.y;
-    }
-    const double div = g[0][0] + g[1][1] + g[2][2];
-    if (prod == PROD_NS) {
-      // H = -(u.grad) u + 0.5 (div u) u   (navier_stokes.py:236-261)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
-        Hs[q][c] = -conv + 0.5 * (div * u[c]);
-      }
-    } else if (prod == PROD_PERT) {
-      // H = -(c u.grad u + U d_x u + v U' e_x) + 0.5 (c div u u + div u U e_x)
-      // (navier_stokes_perturbation.py:160-239)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double conv = u[0] * g[c][0] + u[1] * g[c][1] + u[2] * g[c][2];
-        double hel = clin * conv + U * g[c][0];
-        double dv = clin * (div * u[c]);
-        if (c == 0) {
-          hel += u[1] * dU;
-          dv += div * U;
-        }
-        Hs[q][c] = -hel + 0.5 * dv;
-      }
-    } else {
-      // dual: rows 0..5 adjoint v, rows 6..11 the forward field W (plus U)
-      double wv[3], gw[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double2 r0 = row[6 + 2 * c];
-        const double2 r1 = row[7 + 2 * c];
-        wv[c] = clin * r0.x;
-        gw[c][0] = clin * r0.y;
-        gw[c][1] = clin * r1.x;
-        gw[c][2] = clin * r1.y;
-      }
-      wv[0] += U;
-      gw[0][1] += dU;
-      const double divw = gw[0][0] + gw[1][1] + gw[2][2];
-      // H = -(W.grad) v + [sum_j v_j d_i W_j] + 0.25 ((div W) v + (div v) W)
-      // (navier_stokes_perturbation_dual.py:214-294)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double t1 = wv[0] * g[c][0] + wv[1] * g[c][1] + wv[2] * g[c][2];
-        const double t2 = u[0] * gw[0][c] + u[1] * gw[1][c] + u[2] * gw[2][c];
-        Hs[q][c] = -t1 + t2 + 0.5 * (0.5 * (divw * u[c] + div * wv[c]));
-      }
-    }
+    if (prod == PROD_NS) zmid_products<PROD_NS>(row, U, dU, clin, Hs[q]);
+    else if (prod == PROD_PERT) zmid_products<PROD_PERT>(row, U, dU, clin, Hs[q]);
+    else zmid_products<PROD_DUAL>(row, U, dU, clin, Hs[q]);
   }
   __syncthreads();
   const int R2 = 2 * YB;
@@ -457,6 +469,219 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V
 }
 
 // ===========================================================================
+// FFT passes with the transform length fixed at compile time
+// ===========================================================================
+// Same data flow and arithmetic as the three kernels above; C = Rad<...> is
+// the radix list of the plan.  The host launches these for the lengths of
+// the supported configurations and the run-time kernels otherwise.
+
+template <class C>
+__global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, const double2* __restrict__ tw,
+                                                     const double* __restrict__ kx, const double* __restrict__ kz,
+                                                     int M, int N, int Z, double s_inv) {
+  extern __shared__ double2 lds[];
+  constexpr int P = C::P;
+  const int q0 = blockIdx.x * QB;
+  const int l = blockIdx.y;
+  const int tr = blockIdx.z;
+  const double2* __restrict__ A = set.A[tr];
+  const double2* __restrict__ B = set.B[tr];
+  const int opA = set.opA[tr], opB = set.opB[tr];
+  const int K = M / 2, KZ = Z / 2;
+  for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
+    const int c = w % NSLOT;
+    const int pos = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    double2 val = make_double2(0.0, 0.0);
+    const int n = dit_perm<C, C::NS>(pos);  // bin index
+    int i = -1;
+    if (n <= K) i = n;
+    else if (n >= P - K) i = n - P + M;
+    if (ok && i >= 0) {
+      const int pi = (M - i) % M;
+      const int pj = (Z - j) % Z;
+      const size_t e = ((size_t)i * N + l) * Z + j;
+      const size_t ep = ((size_t)pi * N + l) * Z + pj;
+      const double2 a = apply_op(opA, A[e], kx[i], kz[j]);
+      const double2 ap = apply_op(opA, A[ep], kx[pi], kz[pj]);
+      const double2 b = apply_op(opB, B[e], kx[i], kz[j]);
+      const double2 bp = apply_op(opB, B[ep], kx[pi], kz[pj]);
+      const double2 ha = cherm(a, ap);
+      const double2 hb = cherm(b, bp);
+      val = make_double2((ha.x - hb.y) * s_inv, (ha.y + hb.x) * s_inv);
+    }
+    lds[pos * NSLOT + c] = val;
+  }
+  __syncthreads();
+  fft_lds_ct<C, NSLOT, NSLOT, 1, 256, +1>(lds, tw);
+  double2* __restrict__ T = set.T[tr];
+  for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
+    const int c = w % NSLOT;
+    const int x = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    if (ok) T[((size_t)x * N + l) * Z + j] = lds[x * NSLOT + c];
+  }
+}
+
+// z pass for one (x, y) line: NT packed input rows, products, two packed
+// output rows.  grid (PXlen, N), block zmid_threads(P).
+__host__ __device__ constexpr int zmid_threads(int p) { return p >= 128 ? 192 : 64; }
+
+template <class C, int NT, int PROD>
+__global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double2* __restrict__ T, long tstride,
+                                                                    double2* __restrict__ V, long vstride,
+                                                                    const double2* __restrict__ tw, int N, int Z,
+                                                                    double clin, const double* __restrict__ Ub,
+                                                                    const double* __restrict__ dUb) {
+  extern __shared__ double2 lds[];
+  constexpr int P = C::P;
+  constexpr int NTH = zmid_threads(P);
+  constexpr int ITEMS = (P + NTH - 1) / NTH;
+  const int x = blockIdx.x;
+  const int l = blockIdx.y;
+  const int KZ = Z / 2;
+  const size_t line = ((size_t)x * N + l) * Z;
+  for (int w = threadIdx.x; w < P * NT; w += NTH) lds[w] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int j = threadIdx.x; j < Z; j += NTH) {
+    const int n = (j <= KZ) ? j : P - (Z - j);
+    const int pos = dit_iperm<C, C::NS>(n);
+    double2 vals[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) vals[t] = T[t * tstride + line + j];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) lds[pos * NT + t] = vals[t];
+  }
+  __syncthreads();
+  fft_lds_ct<C, NT, NT, 1, NTH, +1>(lds, tw);
+  const double U = Ub ? Ub[l] : 0.0;
+  const double dU = dUb ? dUb[l] : 0.0;
+  double Hs[ITEMS][3];
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const int z = threadIdx.x + q * NTH;
+    if (z < P) zmid_products<PROD>(lds + z * NT, U, dU, clin, Hs[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const int z = threadIdx.x + q * NTH;
+    if (z < P) {
+      const int pos = dit_iperm<C, C::NS>(z);
+      lds[pos * 2] = make_double2(Hs[q][0], Hs[q][1]);
+      lds[pos * 2 + 1] = make_double2(Hs[q][2], 0.0);
+    }
+  }
+  __syncthreads();
+  fft_lds_ct<C, 2, 2, 1, NTH, -1>(lds, tw);
+  for (int j = threadIdx.x; j < Z; j += NTH) {
+    const int n = (j <= KZ) ? j : P - (Z - j);
+    V[line + j] = lds[n * 2];
+    V[vstride + line + j] = lds[n * 2 + 1];
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(256) void k_fft_x_fwd_t(const double2* __restrict__ V, long vstride,
+                                                     double2* __restrict__ H, long hstride,
+                                                     const double2* __restrict__ tw, int M, int N, int Z,
+                                                     double s_fwd) {
+  extern __shared__ double2 lds[];
+  constexpr int P = C::P;
+  const int q0 = blockIdx.x * QB;
+  const int l = blockIdx.y;
+  const int t = blockIdx.z;
+  const int K = M / 2, KZ = Z / 2;
+  const double2* Vt = V + t * vstride;
+  for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
+    const int c = w % NSLOT;
+    const int pos = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    const int xi = dit_perm<C, C::NS>(pos);
+    lds[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * Z + j] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_lds_ct<C, NSLOT, NSLOT, 1, 256, -1>(lds, tw);
+  auto bin = [&](int i) { return i <= K ? i : P - (M - i); };
+  for (int w = threadIdx.x; w < M * NSLOT; w += 256) {
+    const int c = w % NSLOT;
+    const int i = w / NSLOT;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    if (!ok) continue;
+    int si = i, sc = c;
+    if (i == K + 1 && j == KZ + 1) {
+      si = K;
+      sc = slot_partner(c, q0);
+    } else if (i == K + 1) {
+      sc = slot_partner(c, q0);
+    } else if (j == KZ + 1) {
+      si = (M - i) % M;
+    }
+    const int spi = (M - si) % M;
+    const double2 Wv = lds[bin(si) * NSLOT + sc];
+    const double2 Wp = lds[bin(spi) * NSLOT + slot_partner(sc, q0)];
+    const size_t e = ((size_t)i * N + l) * Z + j;
+    if (t == 0) {
+      const double2 g0 = make_double2(0.5 * (Wv.x + Wp.x), 0.5 * (Wv.y - Wp.y));
+      const double dr = Wv.x - Wp.x, di = Wv.y + Wp.y;  // W - conj(Wp)
+      const double2 g1 = make_double2(0.5 * di, -0.5 * dr);
+      H[e] = cscale(g0, s_fwd);
+      H[hstride + e] = cscale(g1, s_fwd);
+    } else {
+      const double2 g2 = make_double2(0.5 * (Wv.x + Wp.x), 0.5 * (Wv.y - Wp.y));
+      H[2 * hstride + e] = cscale(g2, s_fwd);
+    }
+  }
+}
+
+// lengths with compile-time kernels: x (P' >= 3K+1 from radices 2..8 without 7)
+// and z (powers of two) of the usual mode counts
+typedef Rad<8, 5> RX40;
+typedef Rad<5, 5, 3> RX75;
+typedef Rad<5, 5, 4> RX100;
+typedef Rad<5, 5, 3, 2> RX150;
+typedef Rad<8, 5, 4> RX160;
+typedef Rad<8, 5, 5> RX200;
+typedef Rad<8> RZ8;
+typedef Rad<8, 2> RZ16;
+typedef Rad<8, 4> RZ32;
+typedef Rad<8, 8> RZ64;
+typedef Rad<8, 8, 2> RZ128;
+typedef Rad<8, 8, 4> RZ256;
+
+template <class C>
+static bool plan_is(const FftPlan& f) {
+  if (f.p != C::P || f.nrad != C::NS) return false;
+  for (int s = 0; s < C::NS; ++s)
+    if (f.rad[s] != C::r(s)) return false;
+  return true;
+}
+template <class F>
+static bool visit_x(const FftPlan& f, F&& fn) {
+  if (plan_is<RX200>(f)) return fn(RX200()), true;
+  if (plan_is<RX40>(f)) return fn(RX40()), true;
+  if (plan_is<RX75>(f)) return fn(RX75()), true;
+  if (plan_is<RX100>(f)) return fn(RX100()), true;
+  if (plan_is<RX150>(f)) return fn(RX150()), true;
+  if (plan_is<RX160>(f)) return fn(RX160()), true;
+  return false;
+}
+template <class F>
+static bool visit_z(const FftPlan& f, F&& fn) {
+  if (plan_is<RZ256>(f)) return fn(RZ256()), true;
+  if (plan_is<RZ8>(f)) return fn(RZ8()), true;
+  if (plan_is<RZ16>(f)) return fn(RZ16()), true;
+  if (plan_is<RZ32>(f)) return fn(RZ32()), true;
+  if (plan_is<RZ64>(f)) return fn(RZ64()), true;
+  if (plan_is<RZ128>(f)) return fn(RZ128()), true;
+  return false;
+}
+
+// ===========================================================================
 // linearised nonlinear term without transforms
 // ===========================================================================
 // The physical field of a stored spectrum X is IFFT(Herm(X)); multiplying by a
@@ -520,35 +745,6 @@ __global__ __launch_bounds__(256) void k_nl_lin(const double2* __restrict__ U0, 
 // pencils (+-a, +-b).  Vectors live in LDS as [vec][y], vec = 2*pencil + re/im.
 // Matrices are stored transposed (AT[j*N + i] = A[i][j]) so consecutive
 // threads (rows i) read consecutive addresses.
-struct SolveArgs {
-  double2* U;  // state, 3 components (in place)
-  long cs;
-  const double2* H;      // nonlinear term H (3 components, stored convention)
-  double2* HV;           // out: h_v of this substep (the next substep's old term)
-  double2* HG;           // out: h_g
-  const double2* HVo;
-  const double2* HGo;
-  double2* CONV;         // out: (0,0) line of conv_ns: [0..N) comp 0, [N..2N) comp 2
-  const double2* CONVo;
-  const double* D_T;     // N*N transposed
-  const double* D2_T;
-  const double* RB_T;    // rhs_base[s] transposed
-  const double* LI_T;    // lhs_inv[s] transposed, per |k|
-  const double* PO_T;    // poisson transposed, per |k|
-  const double* VA;      // v_a[s], per |k|
-  const double* DVA;     // D v_a[s], per |k|
-  const double* DVB;     // D flip(v_a)[s], per |k|
-  const double* INFL;    // infl_inv[s], per |k|
-  const double* RNS;     // rhs_ns[s] (2N)^2 row-major
-  const double* LNS;     // lhs_inv_ns[s]
-  const double* kx;
-  const double* kz;
-  int M, N, Z, Ka, Kb;
-  double dt_gamma, dt_xi, diag_alpha;  // dt*gamma_s, dt*xi_s, alpha_s*dt/Re
-  double dpdx;                         // scaled pressure gradient
-  int use_old;
-};
-
 // Per-group matvecs on FP64 MFMA.  The group's vectors live in LDS as
 // [y][SLD] (16 columns: column 2p+r is pencil p, re/im r; a second set of 8
 // for omega_y where needed).  Rows N..JSD_NYMAX-1 stay zero so the K padding
@@ -612,221 +808,7 @@ __device__ void mv_mfma(const double* __restrict__ AT, int N, const double* X, d
   }
 }
 
-__global__ __launch_bounds__(256) void k_solve(SolveArgs S) {
-  // three [y][16] buffers (rows up to JSD_NYMAX) keep three workgroups per CU
-  __shared__ double X16[JSD_NYMAX * SLD];
-  __shared__ double Y16[JSD_NYMAX * SLD];
-  __shared__ double V16[JSD_NYMAX * SLD];
-  __shared__ double sab[16];
-  __shared__ int pis[4], pjs[4];
-  __shared__ int npen;
-
-  const int nblk = gridDim.x;
-  // XCD-aware order: blocks are dispatched round-robin over 8 XCDs;
-  // consecutive logical groups (neighbouring kz) land on the same XCD and
-  // share its L2 lines.
-  const int per = (nblk + 7) / 8;
-  const int g = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (g >= S.Ka * S.Kb) return;
-  const int a = g / S.Kb, b = g % S.Kb;
-  const int M = S.M, N = S.N, Z = S.Z;
-  if (threadIdx.x == 0) {
-    int n = 0;
-    pis[n] = a; pjs[n] = b; ++n;
-    if (a > 0) { pis[n] = M - a; pjs[n] = b; ++n; }
-    if (b > 0) { pis[n] = a; pjs[n] = Z - b; ++n; }
-    if (a > 0 && b > 0) { pis[n] = M - a; pjs[n] = Z - b; ++n; }
-    npen = n;
-  }
-  for (int it = threadIdx.x; it < JSD_NYMAX * SLD; it += blockDim.x) {
-    X16[it] = 0.0;
-    Y16[it] = 0.0;
-    V16[it] = 0.0;
-  }
-  __syncthreads();
-  const int np = npen;
-  const int nv = 2 * np;
-  const double kxa = S.kx[a], kzb = S.kz[b];
-  const double k2 = kxa * kxa + kzb * kzb;
-  const size_t kofs = (size_t)g * N * N;
-
-  // ---- load v (V16 cols 0..7), omega_y = (i kz) u - (i kx) w (X16 cols 8..15),
-  //      X = (i kx) H0 + (i kz) H2 (Y16 cols 0..7), H1 (Y16 cols 8..15);
-  //      h_g = (i kz) H0 - (i kx) H2 is final here   (navier_stokes.py:48-73)
-  for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
-    const int p = it / N, l = it % N;
-    const int i = pis[p], j = pjs[p];
-    const size_t e = ((size_t)i * N + l) * Z + j;
-    const double kxv = S.kx[i], kzv = S.kz[j];
-    const double2 u = S.U[e], v = S.U[S.cs + e], w = S.U[2 * S.cs + e];
-    V16[l * SLD + 2 * p] = v.x;
-    V16[l * SLD + 2 * p + 1] = v.y;
-    const double2 om = csub(cik(kzv, u), cik(kxv, w));
-    X16[l * SLD + 8 + 2 * p] = om.x;
-    X16[l * SLD + 8 + 2 * p + 1] = om.y;
-    const double2 h0 = S.H[e], h1 = S.H[S.cs + e], h2 = S.H[2 * S.cs + e];
-    const double2 xc = cadd(cik(kxv, h0), cik(kzv, h2));
-    Y16[l * SLD + 2 * p] = xc.x;
-    Y16[l * SLD + 2 * p + 1] = xc.y;
-    Y16[l * SLD + 8 + 2 * p] = h1.x;
-    Y16[l * SLD + 8 + 2 * p + 1] = h1.y;
-    S.HG[e] = csub(cik(kzv, h0), cik(kxv, h2));
-    if (i == 0 && j == 0) {
-      S.CONV[l] = make_double2(-h0.x, -h0.y);
-      S.CONV[N + l] = make_double2(-h2.x, -h2.y);
-    }
-  }
-  __syncthreads();
-  // ---- h_v = (-D X + (-kx^2) H1) + (-kz^2) H1
-  mv_mfma(S.D_T, N, Y16, X16, 8, 0, N);
-  __syncthreads();
-  for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
-    const int p = it / N, l = it % N;
-    const int i = pis[p], j = pjs[p];
-    const size_t e = ((size_t)i * N + l) * Z + j;
-    const double kxv = S.kx[i], kzv = S.kz[j];
-    const double2 h1 = make_double2(Y16[l * SLD + 8 + 2 * p], Y16[l * SLD + 8 + 2 * p + 1]);
-    const double2 dx = make_double2(X16[l * SLD + 2 * p], X16[l * SLD + 2 * p + 1]);
-    S.HV[e] = cadd(cadd(cscale(dx, -1.0), cscale(h1, -(kxv * kxv))), cscale(h1, -(kzv * kzv)));
-  }
-  __syncthreads();
-  // ---- phi = ((-kx^2) v + D^2 v) + (-kz^2) v    (navier_stokes.py:1716-1725)
-  mv_mfma(S.D2_T, N, V16, Y16, 8, 0, N);
-  __syncthreads();
-  for (int it = threadIdx.x; it < nv * N; it += blockDim.x) {
-    const int c = it % nv, l = it / nv;
-    const int p = c / 2;
-    const double kxv = S.kx[pis[p]], kzv = S.kz[pjs[p]];
-    const double x = V16[l * SLD + c];
-    X16[l * SLD + c] = ((-(kxv * kxv)) * x + Y16[l * SLD + c]) + (-(kzv * kzv)) * x;
-  }
-  __syncthreads();
-  // ---- rhs = BC(R_s [phi | omega] + dt gamma h + dt xi h_old)
-  //      (navier_stokes.py:1296-1302, :1400-1408)
-  mv_mfma(S.RB_T, N, X16, Y16, 16, 1, N - 1);
-  __syncthreads();
-  for (int it = threadIdx.x; it < N * 2 * np; it += blockDim.x) {
-    const int l = it / (2 * np), r = it % (2 * np);
-    const int p = r / 2, f = r % 2;  // f = 0: v part (h_v), 1: omega part (h_g)
-    double2 vals[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-    if (l > 0 && l < N - 1) {
-      const size_t e = ((size_t)pis[p] * N + l) * Z + pjs[p];
-      double2 h = cscale(f == 0 ? S.HV[e] : S.HG[e], S.dt_gamma);
-      if (S.use_old) h = cadd(h, cscale(f == 0 ? S.HVo[e] : S.HGo[e], S.dt_xi));
-      const int c0 = 8 * f + 2 * p;
-      vals[0] = make_double2((Y16[l * SLD + c0] + S.diag_alpha * (-k2) * X16[l * SLD + c0]) + h.x,
-                             (Y16[l * SLD + c0 + 1] + S.diag_alpha * (-k2) * X16[l * SLD + c0 + 1]) + h.y);
-    }
-    const int c0 = 8 * f + 2 * p;
-    X16[l * SLD + c0] = vals[0].x;
-    X16[l * SLD + c0 + 1] = vals[0].y;
-  }
-  __syncthreads();
-  // ---- [phi' | omega'] = BC(L_s^-1 rhs)
-  mv_mfma(S.LI_T + kofs, N, X16, Y16, 16, 1, N - 1);
-  __syncthreads();
-  if (threadIdx.x < 2 * SLD) Y16[(threadIdx.x < SLD ? 0 : N - 1) * SLD + (threadIdx.x % SLD)] = 0.0;
-  __syncthreads();
-  // ---- v_p = BC(P phi')  (X16 columns 0..7; its wall rows are zero)
-  mv_mfma(S.PO_T + kofs, N, Y16, X16, 8, 1, N - 1);
-  __syncthreads();
-  // ---- D v_p (V16 columns 0..7)
-  mv_mfma(S.D_T, N, X16, V16, 8, 0, N);
-  __syncthreads();
-  // ---- wall correction [a, b] = Minv (-[D v_p]_0, -[D v_p]_{N-1})  (navier_stokes.py:1366-1389)
-  if (threadIdx.x < nv) {
-    const int c = threadIdx.x;
-    const double* mi = S.INFL + (size_t)g * 4;
-    const double r0 = -V16[c], r1 = -V16[(N - 1) * SLD + c];
-    sab[2 * c + 0] = mi[0] * r0 + mi[1] * r1;
-    sab[2 * c + 1] = mi[2] * r0 + mi[3] * r1;
-  }
-  __syncthreads();
-  // ---- v = BC(v_p + a v_a + b flip(v_a)); D v = D v_p + a D v_a + b D flip(v_a);
-  //      u, w from continuity (navier_stokes.py:1506-1527)
-  const double* va = S.VA + (size_t)g * N;
-  const double* dva = S.DVA + (size_t)g * N;
-  const double* dvb = S.DVB + (size_t)g * N;
-  for (int it = threadIdx.x; it < np * N; it += blockDim.x) {
-    const int p = it / N, l = it % N;
-    const int i = pis[p], j = pjs[p];
-    const size_t e = ((size_t)i * N + l) * Z + j;
-    const bool wall = (l == 0 || l == N - 1);
-    const int cr = 2 * p, ci = 2 * p + 1;
-    double2 vnew = make_double2(0.0, 0.0);
-    if (!wall) {
-      vnew.x = (X16[l * SLD + cr] + sab[2 * cr] * va[l]) + sab[2 * cr + 1] * va[N - 1 - l];
-      vnew.y = (X16[l * SLD + ci] + sab[2 * ci] * va[l]) + sab[2 * ci + 1] * va[N - 1 - l];
-    }
-    S.U[S.cs + e] = vnew;
-    if (i == 0 && j == 0) continue;  // mean flow below
-    const double kxv = S.kx[i], kzv = S.kz[j];
-    const double mk2 = -(kxv * kxv + kzv * kzv);
-    const double dr = (V16[l * SLD + cr] + sab[2 * cr] * dva[l]) + sab[2 * cr + 1] * dvb[l];
-    const double di = (V16[l * SLD + ci] + sab[2 * ci] * dva[l]) + sab[2 * ci + 1] * dvb[l];
-    const double orr = wall ? 0.0 : Y16[l * SLD + 8 + cr], oi = wall ? 0.0 : Y16[l * SLD + 8 + ci];
-    // u = (-(i kx) Dv + (i kz) om) / (-k^2); w = (-(i kz) Dv - (i kx) om) / (-k^2)
-    double2 u = make_double2((kxv * di + (-kzv * oi)) / mk2, (-kxv * dr + kzv * orr) / mk2);
-    double2 w = make_double2((kzv * di - (-kxv * oi)) / mk2, (-kzv * dr - kxv * orr) / mk2);
-    if (wall) {
-      u = make_double2(0.0, 0.0);
-      w = make_double2(0.0, 0.0);
-    }
-    S.U[e] = u;
-    S.U[2 * S.cs + e] = w;
-  }
-  if (a != 0 || b != 0) return;
-  __syncthreads();
-  // ---- (0,0) mean flow: [u;w] = Lns^-1 (Rns [u;w] + dt g N_new + dt x N_old)  (navier_stokes.py:1417-1504)
-  double* x2 = V16;  // 2N complex as [re 2N][im 2N]
-  double* y2 = Y16;
-  const int N2 = 2 * N;
-  for (int l = threadIdx.x; l < N2; l += blockDim.x) {
-    const size_t e = ((size_t)(l % N)) * Z;  // (i=0, l, j=0)
-    const double2 val = (l < N) ? S.U[e] : S.U[2 * S.cs + e];
-    x2[l] = val.x;
-    x2[N2 + l] = val.y;
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < N2; r += blockDim.x) {
-    double sr = 0.0, si = 0.0;
-    for (int c = 0; c < N2; ++c) {
-      const double m = S.RNS[(size_t)r * N2 + c];
-      sr += m * x2[c];
-      si += m * x2[N2 + c];
-    }
-    const int comp = r < N ? 0 : 1;
-    const int l = r % N;
-    const double2 cn = S.CONV[comp * N + l];
-    double nr = -cn.x, ni = -cn.y;
-    if (comp == 0) nr += -S.dpdx;
-    double tr = sr + S.dt_gamma * nr, ti = si + S.dt_gamma * ni;
-    if (S.use_old) {
-      const double2 co = S.CONVo[comp * N + l];
-      double orr = -co.x, oi = -co.y;
-      if (comp == 0) orr += -S.dpdx;
-      tr += S.dt_xi * orr;
-      ti += S.dt_xi * oi;
-    }
-    y2[r] = tr;
-    y2[N2 + r] = ti;
-  }
-  __syncthreads();
-  for (int r = threadIdx.x; r < N2; r += blockDim.x) {
-    double sr = 0.0, si = 0.0;
-    for (int c = 0; c < N2; ++c) {
-      const double m = S.LNS[(size_t)r * N2 + c];
-      sr += m * y2[c];
-      si += m * y2[N2 + c];
-    }
-    const int l = r % N;
-    const bool wall = (l == 0 || l == N - 1);
-    const size_t e = (size_t)l * Z;
-    const double2 val = wall ? make_double2(0.0, 0.0) : make_double2(sr, si);
-    if (r < N) S.U[e] = val;
-    else S.U[2 * S.cs + e] = val;
-  }
-}
+#include "jsd_solve.inc"
 
 #include "jsd_adjoint.inc"
 
@@ -904,10 +886,14 @@ struct jsd_plan {
   std::vector<void*> allocs;
   // device tables
   double *DF1, *DF2, *DFT1, *D_R, *D2_R, *RB_R, *LI_R, *PO_R;  // adjoint: D^T tiles, row-major tables
-  double *D_T, *D2_T, *RB_T, *LI_T, *PO_T, *VA, *DVA, *DVB, *INFL, *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
+  double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
+  double *DEO, *DOE, *D2E, *D2O, *RBP, *LIP, *POP, *CW, *RW, *VAP, *VINV;  // parity blocks of the solve
+  double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
+  int nE, nO;
   int *permx, *permz, *ipermx, *ipermz;
   double2 *twx, *twz;
   FftPlan fpx, fpz;
+  int generic_fft;  // 1: run-time-length FFT kernels only (JSD_FFT_GENERIC=1, a diagnostic switch)
   // workspaces
   double2 *DU, *H, *HV, *HG, *HVo, *HGo, *CONV, *CONVo, *T, *V, *WF, *DWF;
   double* partial;
@@ -1025,6 +1011,44 @@ static void tile_D(const double* Dm, int N, int NT, int KS, std::vector<double>&
       }
 }
 
+// ---------------------------------------------------------------------------
+// parity blocks of the wall-normal operators (see jsd_solve.inc)
+// ---------------------------------------------------------------------------
+// Half sizes for N points: interior l = 1..N-2 pairs with l' = N-1-l; the odd
+// half has N/2 - 1 entries, the even half one more when N is odd (centre).
+static inline int half_odd(int N) { return N / 2 - 1; }
+static inline int half_even(int N) { return N / 2 - 1 + (N & 1); }
+
+// Row l (1-based interior or 0 = wall pair) of the block of A mapping the
+// half of parity qi: entry g couples to interior column m = g+1 and its mirror.
+// A is symmetrised first: (A[r][c] + sgn A[N-1-r][N-1-c]) / 2.
+static double sym_entry(const double* A, int N, int sgn, int r, int c) {
+  return 0.5 * (A[(size_t)r * N + c] + sgn * A[(size_t)(N - 1 - r) * N + (N - 1 - c)]);
+}
+static double parity_entry(const double* A, int N, int sgn, int qi, int l, int g) {
+  const int m = g + 1, mp = N - 1 - m;
+  if (m == mp) return sym_entry(A, N, sgn, l, m);
+  return sym_entry(A, N, sgn, l, m) + (qi ? -1.0 : 1.0) * sym_entry(A, N, sgn, l, mp);
+}
+// dense PH x PH block (rows: output half qo, columns: input half qi)
+static void parity_dense(const double* A, int N, int sgn, int qi, int qo, double* blk) {
+  const int no = qo ? half_odd(N) : half_even(N), ni = qi ? half_odd(N) : half_even(N);
+  for (int h = 0; h < PH; ++h)
+    for (int g = 0; g < PH; ++g) blk[h * PH + g] = (h < no && g < ni) ? parity_entry(A, N, sgn, qi, h + 1, g) : 0.0;
+}
+// pre-tiled MFMA A fragments of a dense PH x PH block
+static void parity_tile(const double* blk, double* frag) {
+  for (int t = 0; t < 4; ++t)
+    for (int ks = 0; ks < 16; ++ks)
+      for (int lane = 0; lane < 64; ++lane)
+        frag[(t * 16 + ks) * 64 + lane] = blk[(t * 16 + (lane & 15)) * PH + ks * 4 + (lane >> 4)];
+}
+static void parity_frag(const double* A, int N, int sgn, int qi, int qo, double* frag) {
+  std::vector<double> blk((size_t)PH * PH);
+  parity_dense(A, N, sgn, qi, qo, blk.data());
+  parity_tile(blk.data(), frag);
+}
+
 extern "C" int jsd_version(void) { return 1; }
 extern "C" const char* jsd_last_error(void) { return g_err.c_str(); }
 
@@ -1052,6 +1076,10 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   if (d->kind != JSD_KIND_NS && (!d->base_u || !d->base_du)) JSD_FAIL(JSD_ERR_ARG, "missing base-flow profile");
   jsd_plan* p = new jsd_plan();
   p->d = *d;
+  {
+    const char* env = getenv("JSD_FFT_GENERIC");
+    p->generic_fft = (env && env[0] == '1') ? 1 : 0;
+  }
   p->M = M;
   p->N = N;
   p->Z = Z;
@@ -1089,18 +1117,89 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   TRY(upload(p, &p->DF1, tiled.data(), tiled.size()));
   tile_D(D2.data(), N, p->NT, p->KS, tiled);
   TRY(upload(p, &p->DF2, tiled.data(), tiled.size()));
-  transpose_sq(d->D, tmp.data(), N);
-  TRY(upload(p, &p->D_T, tmp.data(), nn));
-  transpose_sq(D2.data(), tmp.data(), N);
-  TRY(upload(p, &p->D2_T, tmp.data(), nn));
   {
-    std::vector<double> rb(3 * nn), li(3 * nk * nn), po(nk * nn);
-    for (int s = 0; s < 3; ++s) transpose_sq(d->rhs_base + s * nn, rb.data() + s * nn, N);
-    for (size_t k = 0; k < 3 * nk; ++k) transpose_sq(d->lhs_inv + k * nn, li.data() + k * nn, N);
-    for (size_t k = 0; k < nk; ++k) transpose_sq(d->poisson + k * nn, po.data() + k * nn, N);
-    TRY(upload(p, &p->RB_T, rb.data(), rb.size()));
-    TRY(upload(p, &p->LI_T, li.data(), li.size()));
-    TRY(upload(p, &p->PO_T, po.data(), po.size()));
+    // parity blocks: D (centro-antisymmetric), D^2, rhs_base[s], lhs_inv[s][k],
+    // poisson[k] (centro-symmetric), wall rows/columns, v_a in parity form
+    p->nE = half_even(N);
+    p->nO = half_odd(N);
+    if (p->nE > PH) JSD_FAIL(JSD_ERR_ARG, "Ny = %d exceeds the solver's %d-row parity blocks", N, 2 * PH + 2);
+    std::vector<double> f(2 * (size_t)PFRAG);
+    parity_frag(d->D, N, -1, 1, 0, f.data());
+    TRY(upload(p, &p->DEO, f.data(), PFRAG));
+    parity_frag(d->D, N, -1, 0, 1, f.data());
+    TRY(upload(p, &p->DOE, f.data(), PFRAG));
+    parity_frag(D2.data(), N, +1, 0, 0, f.data());
+    parity_frag(D2.data(), N, +1, 1, 1, f.data() + PFRAG);
+    TRY(upload(p, &p->D2E, f.data(), 2 * (size_t)PFRAG));
+    p->D2O = p->D2E + PFRAG;
+    {
+      std::vector<double> rb(3 * 2 * (size_t)PFRAG);
+      for (int s = 0; s < 3; ++s) {
+        parity_frag(d->rhs_base + s * nn, N, +1, 0, 0, rb.data() + (size_t)s * 2 * PFRAG);
+        parity_frag(d->rhs_base + s * nn, N, +1, 1, 1, rb.data() + (size_t)s * 2 * PFRAG + PFRAG);
+      }
+      TRY(upload(p, &p->RBP, rb.data(), rb.size()));
+    }
+    {
+      std::vector<double> li(3 * nk * 2 * (size_t)PFRAG), po(nk * 2 * (size_t)PFRAG);
+      for (size_t k = 0; k < 3 * nk; ++k) {
+        parity_frag(d->lhs_inv + k * nn, N, +1, 0, 0, li.data() + k * 2 * PFRAG);
+        parity_frag(d->lhs_inv + k * nn, N, +1, 1, 1, li.data() + k * 2 * PFRAG + PFRAG);
+      }
+      for (size_t k = 0; k < nk; ++k) {
+        parity_frag(d->poisson + k * nn, N, +1, 0, 0, po.data() + k * 2 * PFRAG);
+        parity_frag(d->poisson + k * nn, N, +1, 1, 1, po.data() + k * 2 * PFRAG + PFRAG);
+      }
+      TRY(upload(p, &p->LIP, li.data(), li.size()));
+      TRY(upload(p, &p->POP, po.data(), po.size()));
+    }
+    // wall columns (input = wall pair (0, N-1)) and the wall row of D
+    std::vector<double> cw(4 * PH, 0.0), rw(2 * PH, 0.0);
+    for (int h = 0; h < p->nE; ++h) {
+      const int l = h + 1;
+      cw[0 * PH + h] = sym_entry(d->D, N, -1, l, 0) - sym_entry(d->D, N, -1, l, N - 1);
+      cw[2 * PH + h] = sym_entry(D2.data(), N, +1, l, 0) + sym_entry(D2.data(), N, +1, l, N - 1);
+      rw[1 * PH + h] = parity_entry(d->D, N, -1, 0, 0, h);
+    }
+    for (int h = 0; h < p->nO; ++h) {
+      const int l = h + 1;
+      cw[1 * PH + h] = sym_entry(d->D, N, -1, l, 0) + sym_entry(d->D, N, -1, l, N - 1);
+      cw[3 * PH + h] = sym_entry(D2.data(), N, +1, l, 0) - sym_entry(D2.data(), N, +1, l, N - 1);
+      rw[0 * PH + h] = parity_entry(d->D, N, -1, 1, 0, h);
+    }
+    TRY(upload(p, &p->CW, cw.data(), cw.size()));
+    TRY(upload(p, &p->RW, rw.data(), rw.size()));
+    // v_a[s][k] in parity form, its derivative and the diagonal influence inverses
+    std::vector<double> deo((size_t)PH * PH), doe((size_t)PH * PH);
+    parity_dense(d->D, N, -1, 1, 0, deo.data());
+    parity_dense(d->D, N, -1, 0, 1, doe.data());
+    std::vector<double> vap(3 * nk * 4 * (size_t)PH, 0.0), vinv(3 * nk * 2);
+    for (size_t k = 0; k < 3 * nk; ++k) {
+      const double* va = d->v_a + k * N;
+      double* o = vap.data() + k * 4 * PH;
+      for (int h = 0; h < p->nE; ++h) {
+        const int l = h + 1, lm = N - 1 - l;
+        o[h] = (l == lm) ? va[l] : 0.5 * (va[l] + va[lm]);
+        if (h < p->nO) o[PH + h] = 0.5 * (va[l] - va[lm]);
+      }
+      long double se = 0.0L, so = 0.0L;
+      for (int h = 0; h < PH; ++h) {
+        long double a1 = 0.0L, a2 = 0.0L;
+        for (int g2 = 0; g2 < PH; ++g2) {
+          a1 += (long double)deo[(size_t)h * PH + g2] * o[PH + g2];
+          a2 += (long double)doe[(size_t)h * PH + g2] * o[g2];
+        }
+        o[2 * PH + h] = (double)a1;
+        o[3 * PH + h] = (double)a2;
+        se += (long double)rw[1 * PH + h] * o[h];
+        so += (long double)rw[0 * PH + h] * o[PH + h];
+      }
+      if (se == 0.0L || so == 0.0L) JSD_FAIL(JSD_ERR_ARG, "singular influence matrix at table %zu", k);
+      vinv[2 * k] = (double)(1.0L / se);
+      vinv[2 * k + 1] = (double)(1.0L / so);
+    }
+    TRY(upload(p, &p->VAP, vap.data(), vap.size()));
+    TRY(upload(p, &p->VINV, vinv.data(), vinv.size()));
   }
   p->DFT1 = p->D_R = p->D2_R = p->RB_R = p->LI_R = p->PO_R = nullptr;
   if (d->kind == JSD_KIND_PERTURBATION && d->linearise) {
@@ -1116,28 +1215,21 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     TRY(upload(p, &p->LI_R, d->lhs_inv, 3 * nk * nn));
     TRY(upload(p, &p->PO_R, d->poisson, nk * nn));
   }
-  TRY(upload(p, &p->VA, d->v_a, 3 * nk * N));
-  {
-    // D v_a and D flip(v_a) per (substep, |k|) for the wall-corrected derivative
-    std::vector<double> dva(3 * nk * N), dvb(3 * nk * N);
-    for (size_t k = 0; k < 3 * nk; ++k) {
-      const double* va = d->v_a + k * N;
-      for (int r = 0; r < N; ++r) {
-        double sa = 0.0, sb = 0.0;
-        for (int c = 0; c < N; ++c) {
-          sa += d->D[(size_t)r * N + c] * va[c];
-          sb += d->D[(size_t)r * N + c] * va[N - 1 - c];
-        }
-        dva[k * N + r] = sa;
-        dvb[k * N + r] = sb;
-      }
-    }
-    TRY(upload(p, &p->DVA, dva.data(), dva.size()));
-    TRY(upload(p, &p->DVB, dvb.data(), dvb.size()));
+  p->VA = p->INFL = nullptr;
+  if (d->kind == JSD_KIND_PERTURBATION && d->linearise) {
+    TRY(upload(p, &p->VA, d->v_a, 3 * nk * N));
+    TRY(upload(p, &p->INFL, d->infl_inv, 3 * nk * 4));
   }
-  TRY(upload(p, &p->INFL, d->infl_inv, 3 * nk * 4));
-  TRY(upload(p, &p->RNS, d->rhs_ns, 3 * 4 * nn));
-  TRY(upload(p, &p->LNS, d->lhs_inv_ns, 3 * 4 * nn));
+  {
+    // mean-flow blocks transposed: the solve reads them column-wise (coalesced)
+    std::vector<double> rt(3 * 4 * nn), lt(3 * 4 * nn);
+    for (int s = 0; s < 3; ++s) {
+      transpose_sq(d->rhs_ns + s * 4 * nn, rt.data() + s * 4 * nn, 2 * N);
+      transpose_sq(d->lhs_inv_ns + s * 4 * nn, lt.data() + s * 4 * nn, 2 * N);
+    }
+    TRY(upload(p, &p->RNS, rt.data(), rt.size()));
+    TRY(upload(p, &p->LNS, lt.data(), lt.size()));
+  }
   TRY(upload(p, &p->kx, d->kx, M));
   TRY(upload(p, &p->kz, d->kz, Z));
   TRY(upload(p, &p->wy, d->y_weights, N));
@@ -1260,23 +1352,50 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     }
   }
   for (int t = 0; t < ntr; ++t) set.T[t] = p->T + t * ts;
+  const bool ct = !p->generic_fft;
   prof_begin(p, 1, st);
-  hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N, ntr), dim3(256), lds_x, st, set, p->fpx, p->kx, p->kz, M, N, Z,
-                     p->s_inv);
+  const bool xi_ct = ct && visit_x(p->fpx, [&](auto c) {
+    hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, dim3(nqb, N, ntr), dim3(256), lds_x, st, set, p->fpx.tw, p->kx,
+                       p->kz, M, N, Z, p->s_inv);
+  });
+  if (!xi_ct)
+    hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N, ntr), dim3(256), lds_x, st, set, p->fpx, p->kx, p->kz, M, N, Z,
+                       p->s_inv);
   prof_end(p, 1, st);
   HIPCHK(hipGetLastError());
-  // y planes per z-pass workgroup: LDS <= 80 KB and <= ZMID_ITEMS items per thread
-  int YB = (int)((80 * 1024) / ((size_t)d.ppz * ntr * sizeof(double2)));
-  YB = std::max(1, std::min(YB, std::min(4, (ZMID_ITEMS * 256) / d.ppz)));
-  const size_t lds_z = (size_t)d.ppz * ntr * YB * sizeof(double2);
   prof_begin(p, 2, st);
-  hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, (N + YB - 1) / YB), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V,
-                     (long)ts, p->fpz, N, Z, prod, clin, p->Ub, p->dUb, YB);
+  const bool z_ct = ct && visit_z(p->fpz, [&](auto c) {
+    using C = decltype(c);
+    const size_t lds = (size_t)C::P * ntr * sizeof(double2);
+    const dim3 grid(d.ppx, N), block(zmid_threads(C::P));
+    if (prod == PROD_NS)
+      hipLaunchKernelGGL((k_fft_z_mid_t<C, 6, PROD_NS>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
+                         p->fpz.tw, N, Z, clin, p->Ub, p->dUb);
+    else if (prod == PROD_PERT)
+      hipLaunchKernelGGL((k_fft_z_mid_t<C, 6, PROD_PERT>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
+                         p->fpz.tw, N, Z, clin, p->Ub, p->dUb);
+    else
+      hipLaunchKernelGGL((k_fft_z_mid_t<C, 12, PROD_DUAL>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
+                         p->fpz.tw, N, Z, clin, p->Ub, p->dUb);
+  });
+  if (!z_ct) {
+    // y planes per z-pass workgroup: LDS <= 80 KB and <= ZMID_ITEMS items per thread
+    int YB = (int)((80 * 1024) / ((size_t)d.ppz * ntr * sizeof(double2)));
+    YB = std::max(1, std::min(YB, std::min(4, (ZMID_ITEMS * 256) / d.ppz)));
+    const size_t lds_z = (size_t)d.ppz * ntr * YB * sizeof(double2);
+    hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, (N + YB - 1) / YB), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V,
+                       (long)ts, p->fpz, N, Z, prod, clin, p->Ub, p->dUb, YB);
+  }
   prof_end(p, 2, st);
   HIPCHK(hipGetLastError());
   prof_begin(p, 3, st);
-  hipLaunchKernelGGL(k_fft_x_fwd, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx, M,
-                     N, Z, p->s_fwd);
+  const bool xf_ct = ct && visit_x(p->fpx, [&](auto c) {
+    hipLaunchKernelGGL(k_fft_x_fwd_t<decltype(c)>, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H,
+                       (long)fs, p->fpx.tw, M, N, Z, p->s_fwd);
+  });
+  if (!xf_ct)
+    hipLaunchKernelGGL(k_fft_x_fwd, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx,
+                       M, N, Z, p->s_fwd);
   prof_end(p, 3, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
@@ -1300,15 +1419,18 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   S.HGo = p->HGo;
   S.CONV = p->CONV;
   S.CONVo = p->CONVo;
-  S.D_T = p->D_T;
-  S.D2_T = p->D2_T;
-  S.RB_T = p->RB_T + s * nn;
-  S.LI_T = p->LI_T + (size_t)s * nk * nn;
-  S.PO_T = p->PO_T;
-  S.VA = p->VA + (size_t)s * nk * p->N;
-  S.DVA = p->DVA + (size_t)s * nk * p->N;
-  S.DVB = p->DVB + (size_t)s * nk * p->N;
-  S.INFL = p->INFL + (size_t)s * nk * 4;
+  S.DEO = p->DEO;
+  S.DOE = p->DOE;
+  S.D2E = p->D2E;
+  S.D2O = p->D2O;
+  S.RBE = p->RBP + (size_t)s * 2 * PFRAG;
+  S.RBO = S.RBE + PFRAG;
+  S.LIP = p->LIP + (size_t)s * nk * 2 * PFRAG;
+  S.POP = p->POP;
+  S.CW = p->CW;
+  S.RW = p->RW;
+  S.VAP = p->VAP + (size_t)s * nk * 4 * PH;
+  S.VINV = p->VINV + (size_t)s * nk * 2;
   S.RNS = p->RNS + (size_t)s * 4 * nn;
   S.LNS = p->LNS + (size_t)s * 4 * nn;
   S.kx = p->kx;
@@ -1318,6 +1440,8 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   S.Z = p->Z;
   S.Ka = p->Ka;
   S.Kb = p->Kb;
+  S.nE = p->nE;
+  S.nO = p->nO;
   const double dt = p->d.dt;
   S.dt_gamma = dt * gam[s];
   S.dt_xi = dt * xi[s];

@@ -1,0 +1,99 @@
+// jsd_fft_ct.h - LDS FFT with the transform length and radices fixed at
+// compile time (gfx950, FP64).
+//
+// Same algorithm and arithmetic as the run-time fft_lds of jsd_kernels.h
+// (in-place mixed-radix decimation in time, input loaded in digit-reversed
+// order, correctly-rounded twiddle table exp(-2 pi i m / p)); with every
+// length, stride and radix a constant the index arithmetic reduces to
+// multiply-shift sequences, the twiddle index needs no modulo
+// (m * k1 * stride < p) and each stage's work loop has a constant trip count.
+#pragma once
+#include "jsd_kernels.h"
+
+// radices of the stages, innermost (first DIT stage) first; 1 = no stage
+template <int R0, int R1 = 1, int R2 = 1, int R3 = 1>
+struct Rad {
+  static constexpr int P = R0 * R1 * R2 * R3;
+  static constexpr int NS = 1 + (R1 > 1) + (R2 > 1) + (R3 > 1);
+  static constexpr int r(int s) { return s == 0 ? R0 : s == 1 ? R1 : s == 2 ? R2 : R3; }
+  static constexpr int len(int k) { return k <= 0 ? 1 : r(k - 1) * len(k - 1); }  // product of r(0..k-1)
+};
+
+// LDS position -> input index of the DIT order over stages 0..K-1:
+// order(len) = [r * sub[j] + n2 for n2 in 0..r-1 for j], r the outermost radix.
+template <class C, int K>
+__device__ __forceinline__ int dit_perm(int q) {
+  if constexpr (K == 0) {
+    return 0;
+  } else {
+    constexpr int r = C::r(K - 1);
+    constexpr int Q = C::len(K - 1);
+    return r * dit_perm<C, K - 1>(q % Q) + q / Q;
+  }
+}
+// input index -> LDS position
+template <class C, int K>
+__device__ __forceinline__ int dit_iperm(int n) {
+  if constexpr (K == 0) {
+    return 0;
+  } else {
+    constexpr int r = C::r(K - 1);
+    constexpr int Q = C::len(K - 1);
+    return (n % r) * Q + dit_iperm<C, K - 1>(n / r);
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_ct(double2* v) {
+  constexpr double sign = (double)SIGN;
+  if constexpr (R == 2) dft2(v);
+  else if constexpr (R == 4) dft4(v, sign);
+  else if constexpr (R == 8) dft8(v, sign);
+  else dft_generic<R>(v, sign);
+}
+
+// One DIT stage (radix R, partial length L) on NROWS rows;
+// element (row, pos) at buf[pos * PS + row * RS].
+template <int R, int P, int L, int NROWS, int PS, int RS, int NTH, int SIGN>
+__device__ __forceinline__ void fft_stage_ct(double2* buf, const double2* __restrict__ tw) {
+  constexpr int Lp = L / R;
+  constexpr int nb = P / R;
+  constexpr int stride = P / L;
+  constexpr int total = nb * NROWS;
+#pragma unroll 2
+  for (int w = threadIdx.x; w < total; w += NTH) {
+    const int row = w % NROWS;
+    const int b = w / NROWS;
+    const int g = b / Lp, k1 = b - g * Lp;
+    const int base = g * L + k1;
+    double2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      double2 x = buf[(base + m * Lp) * PS + row * RS];
+      if (Lp > 1 && m > 0 && k1 > 0) {
+        double2 t = tw[m * k1 * stride];
+        if (SIGN > 0) t.y = -t.y;
+        x = cmul(x, t);
+      }
+      v[m] = x;
+    }
+    dft_ct<R, SIGN>(v);
+#pragma unroll
+    for (int m = 0; m < R; ++m) buf[(base + m * Lp) * PS + row * RS] = v[m];
+  }
+}
+
+template <class C, int S, int NROWS, int PS, int RS, int NTH, int SIGN>
+__device__ __forceinline__ void fft_stages_ct(double2* buf, const double2* __restrict__ tw) {
+  if constexpr (S < C::NS) {
+    fft_stage_ct<C::r(S), C::P, C::len(S + 1), NROWS, PS, RS, NTH, SIGN>(buf, tw);
+    __syncthreads();
+    fft_stages_ct<C, S + 1, NROWS, PS, RS, NTH, SIGN>(buf, tw);
+  }
+}
+
+// Full in-place FFT of NROWS rows already in DIT order; SIGN -1 forward, +1 inverse.
+template <class C, int NROWS, int PS, int RS, int NTH, int SIGN>
+__device__ __forceinline__ void fft_lds_ct(double2* buf, const double2* __restrict__ tw) {
+  fft_stages_ct<C, 0, NROWS, PS, RS, NTH, SIGN>(buf, tw);
+}

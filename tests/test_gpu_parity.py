@@ -49,6 +49,43 @@ def test_nonlinear_ns(shape, lx, lz):
     assert rel(conv, rc) < 1e-12
 
 
+@pytest.mark.parametrize("shape", [(8, 33, 8), (16, 33, 16)])
+def test_nonlinear_ns_runtime_lengths(shape):
+    """x transform lengths 15 and 25 have no compile-time kernel: the
+    run-time-length FFT kernels run here."""
+    g, d = pair(shape)
+    assert g.ppx in (15, 25)
+    plan = StepPlan(g, 5000.0, 0.005, KIND_NS)
+    u = ns_field(g, seed=7)
+    hv, hg, conv = plan.nonlinear(u.to(DEV))
+    rv, rg, _, rc = O.nl_ns_skew(d, u)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
+@pytest.mark.parametrize("kind", [KIND_NS, KIND_DUAL])
+def test_compile_time_and_runtime_fft_agree(monkeypatch, kind):
+    """The compile-time-length FFT kernels and the run-time-length ones
+    (forced with JSD_FFT_GENERIC=1 at plan creation) do the same arithmetic."""
+    g, _ = pair((24, 33, 16), 2 * np.pi, np.pi)
+    u = perturbation_field(g, seed=8, energy=1e-2).to(DEV)
+    uf = perturbation_field(g, seed=9, energy=1e-2).to(DEV)
+
+    def make():
+        if kind == KIND_NS:
+            return StepPlan(g, 5000.0, 0.005, KIND_NS)
+        fwd_ops = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION).ops
+        return StepPlan(g, 3000.0, 0.01, KIND_DUAL, ops=fwd_ops, explicit_dt=-0.01, plan_re=-3000.0)
+
+    monkeypatch.delenv("JSD_FFT_GENERIC", raising=False)
+    a = make().nonlinear(u, None if kind == KIND_NS else uf)
+    monkeypatch.setenv("JSD_FFT_GENERIC", "1")
+    b = make().nonlinear(u, None if kind == KIND_NS else uf)
+    for x, y in zip(a, b):
+        assert rel(x, y) < 1e-14
+
+
 @pytest.mark.parametrize("linearise", [True, False])
 @pytest.mark.parametrize("shape,lx,lz", SHAPES)
 def test_nonlinear_perturbation(shape, lx, lz, linearise):
