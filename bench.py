@@ -30,12 +30,14 @@ sys.path.insert(0, REPO)
 
 METRIC = "DNS steps/s at 128×129×128 FP64; batched fwd+adjoint gradient evals/s vs plain"
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6
+TRAFFIC_FILE = "hbm_traffic_ns_128x129x128.json"
 CLASSES = ("ygemm", "fft_x_inv", "fft_z_mid", "fft_x_fwd", "ygemm_hv", "solve", "nl_lin")
 KERNEL_NAMES = {
     "ygemm": "k_ygemm<0>",
-    "fft_x_inv": "k_fft_x_inv",
-    "fft_z_mid": "k_fft_z_mid",
-    "fft_x_fwd": "k_fft_x_fwd",
+    "fft_x_inv": "k_fft_x_inv_t<Rad<8, 5, 5, 1> >",
+    "fft_z_mid": "k_fft_z_mid_t<Rad<8, 8, 4, 1>, 6, 0>",
+    "fft_x_fwd": "k_fft_x_fwd_t<Rad<8, 5, 5, 1> >",
     "ygemm_hv": "k_ygemm<1>",
     "solve": "k_solve",
     "nl_lin": "k_nl_lin",
@@ -46,19 +48,122 @@ def algorithmic_bytes(g, nsub: int = 3) -> dict:
     """Bytes one launch of each kernel class must move (inputs read once,
     outputs written once), averaged over the substeps of a step.
     S = one complex128 spectral field, T = one complex128 x-transformed
-    intermediate (ppx x Ny x Nz_modes), F = one real per-|k| operator family."""
+    intermediate (ppx x Ny x Nz_modes).  The solve reads the substep's
+    Helmholtz-inverse and the Poisson even/odd blocks (nE^2 + nO^2 doubles per
+    |k| each), the v_a parity tables (4 x 64 + 2 doubles per |k|) and
+    12.33 S of state: u, v, w, H, on average 4/3 old-term fields; it writes
+    u, v, w, h_v, h_g."""
     S = 16.0 * g.nxm * g.ny * g.nzm
     T = 16.0 * g.ppx * g.ny * g.nzm
-    F = 8.0 * g.ny * g.ny * g.ka * g.kb
-    small = 8.0 * g.ka * g.kb * (g.ny + 4)
+    nO = g.ny // 2 - 1
+    nE = nO + (g.ny & 1)
+    nk = g.ka * g.kb
+    tables = 2.0 * nk * (nE * nE + nO * nO) * 8.0 + nk * (4 * 64 + 2) * 8.0
     return {
-        "ygemm": 6 * S,                      # read 3 components, write 3
-        "fft_x_inv": 9 * S + 6 * T,          # 6 packed transforms: read 9 source fields, write 6 T
-        "fft_z_mid": 8 * T,                  # read 6 T, write 2 T
-        "fft_x_fwd": 2 * T + 3 * S,          # read 2 T, write 3 components of H
-        "ygemm_hv": 3 * S + 2 * S,           # read H, write h_v, h_g
-        "solve": 2 * F + small + (3 + 2 + 2 * (nsub - 1) / nsub + 3) * S,  # tables, state, h(+old), state
+        "ygemm": 6 * S,  # read 3 components, write 3
+        "fft_x_inv": 6 * S + 6 * T,  # read u_c and D u_c, write 6 packed transforms
+        "fft_z_mid": 8 * T,  # read 6 T, write 2 T
+        "fft_x_fwd": 2 * T + 3 * S,  # read 2 T, write 3 components of H
+        "ygemm_hv": 3 * S + 2 * S,  # read H, write h_v, h_g
+        "solve": tables + (3 + 3 + 2.0 * (nsub - 1) / nsub + 3 + 2) * S,
         "nl_lin": 4 * S + 3 * S,
+    }
+
+
+def ygemm_flops(g, ncomp: int = 3) -> float:
+    """FP64 flops of one y-derivative launch: D (Ny x Ny) times the
+    Ny x (2 Nz_modes) real matrix of each kx row, for ncomp components."""
+    return 2.0 * g.ny * g.ny * (2 * g.nzm) * g.nxm * ncomp
+
+
+BATCH_WORKLOAD = (
+    "64 linearised perturbation runs about Poiseuille flow, seed 0: Re~U[500,6000], alpha~U[0.5,2], "
+    "beta~U[0,2], Ny in {65,97,129}, (Nx,Nz) in {(16,16),(32,16),(48,32)}, dt 0.01, energy 1e-6"
+)
+
+
+def batch_bench(nsteps: int) -> dict:
+    """Batched forward steps and gain-gradient evaluations of 64 uneven
+    linearised runs against the same runs advanced one at a time on the GPU."""
+    import torch
+
+    from jax_spectral_dns_amd import KIND_PERTURBATION
+    from jax_spectral_dns_amd import domain_math as DM
+    from jax_spectral_dns_amd import initial
+    from jax_spectral_dns_amd.batch import BatchGradient, BatchPlan
+    from jax_spectral_dns_amd.plan import StepPlan
+    from jax_spectral_dns_amd.vjp import gain_and_grad
+
+    rng = np.random.default_rng(0)
+    shapes = [(16, 16), (32, 16), (48, 32)]
+    plans, u0 = [], []
+    t0 = time.time()
+    for r in range(64):
+        re = rng.uniform(500.0, 6000.0)
+        alpha = rng.uniform(0.5, 2.0)
+        beta = rng.uniform(0.0, 2.0)
+        ny = int(rng.choice([65, 97, 129]))
+        nx, nz = shapes[int(rng.integers(0, 3))]
+        lz = 2 * np.pi / beta if beta >= 1e-3 else 1e-3
+        g = DM.make_grid((nx, ny, nz), 2 * np.pi / alpha, lz)
+        plans.append(StepPlan(g, re, 0.01, KIND_PERTURBATION, linearise=True))
+        plans[-1].ops = None  # the host copies of the tables are not needed once on the device
+        u0.append(torch.from_numpy(initial.perturbation(g, seed=r, energy=1e-6)).cuda())
+    t_build = time.time() - t0
+    dev_sync = torch.cuda.synchronize
+
+    # forward steps: one run at a time, then batched
+    us = [u.clone() for u in u0]
+    dev_sync()
+    t0 = time.perf_counter()
+    for p, u in zip(plans, us):
+        p.step(u, 0.0, nsteps)
+    dev_sync()
+    t_one = time.perf_counter() - t0
+    ub = [u.clone() for u in u0]
+    t0 = time.time()
+    batch = BatchPlan(plans, ub)
+    t_tune = time.time() - t0
+    info = batch.info()
+    dev_sync()
+    t0 = time.perf_counter()
+    batch.step(nsteps)
+    dev_sync()
+    t_bat = time.perf_counter() - t0
+    fwd_diff = max(float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) for a, b in zip(ub, us))
+    batch.close()
+
+    # gradient evaluations J = E(T)/E(0), dJ/du0
+    dev_sync()
+    t0 = time.perf_counter()
+    ref = [gain_and_grad(u, p, nsteps, checkpoint_every=50) for p, u in zip(plans, u0)]
+    dev_sync()
+    tg_one = time.perf_counter() - t0
+    bg = BatchGradient(plans)
+    dev_sync()
+    t0 = time.perf_counter()
+    J, G = bg(u0, nsteps)
+    dev_sync()
+    tg_bat = time.perf_counter() - t0
+    bg.close()
+    gdiff = max(float(torch.linalg.norm(a - b[1]) / torch.linalg.norm(b[1])) for a, b in zip(G, ref))
+    jdiff = max(abs(a - b[0]) / abs(b[0]) for a, b in zip(J, ref))
+    nrun = len(plans)
+    return {
+        "workload": BATCH_WORKLOAD + ", %d steps" % nsteps,
+        "steps_per_s_batched": nrun * nsteps / t_bat,
+        "steps_per_s_one_at_a_time": nrun * nsteps / t_one,
+        "step_speedup": t_one / t_bat,
+        "grad_evals_per_s_batched": nrun / tg_bat,
+        "grad_evals_per_s_one_at_a_time": nrun / tg_one,
+        "grad_speedup": tg_one / tg_bat,
+        "max_rel_diff_state": fwd_diff,
+        "max_rel_diff_grad": gdiff,
+        "max_rel_diff_gain": jdiff,
+        "autotuned_grid": {k: v["grid"] for k, v in info.items()},
+        "logical_blocks": {k: v["blocks"] for k, v in info.items()},
+        "plan_build_s": round(t_build, 2),
+        "autotune_and_capture_s": round(t_tune, 2),
     }
 
 
@@ -69,6 +174,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-runs figure")
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument(
         "--copy-probe",
         action="store_true",
@@ -139,10 +246,16 @@ def main() -> int:
     dom_ms = ms[di] / max(1, cnt[di])
     nbytes = algorithmic_bytes(g)[dominant]
     achieved = nbytes / (dom_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE (calibrated
+    # against a copy kernel of known size) for this workload
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "traffic_%s_cfg1.json" % dominant)
+    tpath = os.path.join(REPO, "profiles", TRAFFIC_FILE)
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        entry = json.load(open(tpath)).get("kernels", {}).get(dominant)
+        if entry is not None:
+            traffic = entry.get("bytes_per_launch")
+    yi = CLASSES.index("ygemm")
+    ygemm_ms = ms[yi] / max(1, cnt[yi])
     step_ms = 1e3 * elapsed / args.steps
     finite = bool(torch.isfinite(u).all())
 
@@ -180,6 +293,14 @@ def main() -> int:
             "algorithmic_bytes_per_launch": nbytes,
         },
         "kernel_ms_per_step": {c: ms[i] / args.steps for i, c in enumerate(CLASSES)},
+        "mfma_ygemm": {
+            "kernel": KERNEL_NAMES["ygemm"],
+            "achieved": ygemm_flops(g) / (ygemm_ms * 1e-3) / 1e12,
+            "peak": FP64_MFMA_PEAK_TFS,
+            "unit": "TFLOP/s",
+            "frac": ygemm_flops(g) / (ygemm_ms * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFS,
+            "launch_ms": ygemm_ms,
+        },
     }
 
     if not args.no_cpu_baseline:
@@ -209,6 +330,11 @@ def main() -> int:
             % args.cpu_steps,
         }
         out["parity_rel_l2_vs_oracle"] = diff
+    if not args.no_batch:
+        plan.close()
+        del u
+        torch.cuda.empty_cache()
+        out["batch"] = batch_bench(args.batch_steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -119,6 +119,37 @@ int jsd_ydiff(jsd_plan* plan, const void* in, void* out, int ncomp, int order, v
  * then resets them. */
 int jsd_profile(jsd_plan* plan, int enable, double* ms, long* count);
 
+/* ---- batches of independent runs ------------------------------------------
+ * Many linearised runs (JSD_KIND_PERTURBATION or JSD_KIND_DUAL plans with
+ * linearise = 1, uneven shapes, Re, dt and box sizes allowed) advanced with
+ * one launch per stage for all of them: each stage kernel walks the
+ * concatenated per-run grids with a persistent grid whose size is picked by
+ * timing candidates at creation (the autotuner), and one RK3 step (forward
+ * and, for perturbation plans, adjoint) is captured once in a HIP graph and
+ * replayed.  Replaces a Python loop of perform_time_step calls over runs
+ * (navier_stokes.py:1778-1813, one launch sequence per run and step) for the
+ * batched gradient evaluations of the optimisation drivers
+ * (gradient_descent_solver.py:490-934); the batching itself has no
+ * counterpart in the reference.
+ *
+ * states[r] / adjoints[r] are the runs' device state buffers (complex128,
+ * each plan's layout), bound at creation; adjoints may be NULL (no VJP).
+ * Creation runs the autotuner on the stream and leaves the states
+ * unchanged.  A batch uses its plans' workspaces: do not step those plans
+ * concurrently with the batch. */
+typedef struct jsd_batch jsd_batch;
+int jsd_batch_create(jsd_plan* const* plans, void* const* states, void* const* adjoints, int nrun, void* stream,
+                     jsd_batch** out);
+int jsd_batch_destroy(jsd_batch* batch);
+/* nsteps RK3 steps of every run (as jsd_step with dPdx = 0 on each). */
+int jsd_batch_step(jsd_batch* batch, int nsteps, void* stream);
+/* nsteps adjoint steps of every run (as jsd_step_vjp on each adjoint). */
+int jsd_batch_step_vjp(jsd_batch* batch, int nsteps, void* stream);
+/* Autotuned persistent grid sizes of the 7 stage kernels (forward y-GEMM,
+ * linearised term, solve; adjoint solve, h_v/h_g transpose, linearised term
+ * transpose, D^T accumulation) and their total logical block counts. */
+int jsd_batch_info(jsd_batch* batch, int* grid, int* blocks);
+
 const char* jsd_last_error(void);
 int jsd_version(void);
 

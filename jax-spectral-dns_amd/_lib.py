@@ -68,6 +68,11 @@ EXPORTS = (
     "jsd_step_vjp",
     "jsd_energy_grad",
     "jsd_profile",
+    "jsd_batch_create",
+    "jsd_batch_destroy",
+    "jsd_batch_step",
+    "jsd_batch_step_vjp",
+    "jsd_batch_info",
     "jsd_last_error",
     "jsd_version",
 )
@@ -91,6 +96,12 @@ def load() -> ctypes.CDLL:
     lib.jsd_step_vjp.argtypes = [vp, vp, ctypes.c_int, vp]
     lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
+    lib.jsd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, vp,
+                                     ctypes.POINTER(vp)]
+    lib.jsd_batch_destroy.argtypes = [vp]
+    lib.jsd_batch_step.argtypes = [vp, ctypes.c_int, vp]
+    lib.jsd_batch_step_vjp.argtypes = [vp, ctypes.c_int, vp]
+    lib.jsd_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.jsd_last_error.restype = ctypes.c_char_p
     lib.jsd_version.restype = ctypes.c_int
     for name in EXPORTS:

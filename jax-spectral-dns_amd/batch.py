@@ -1,0 +1,122 @@
+"""Batches of independent linearised runs on one GPU.
+
+``BatchPlan`` binds a list of ``StepPlan`` objects (linearised perturbation
+or dual plans of any shape, Re, dt and box) to their device state buffers and
+advances all of them with one launch per stage (``jsd_batch_step``): the
+stage kernels walk the concatenated per-run grids with an autotuned
+persistent grid and one RK3 step is replayed from a captured HIP graph.  With
+adjoint buffers it also runs the batched discrete adjoint
+(``jsd_batch_step_vjp``).  The reference advances such runs one at a time
+(a loop of perform_time_step, navier_stokes.py:1778-1813, per run);
+``batch_gain_and_grad`` is the batched form of ``vjp.gain_and_grad`` for the
+gradient evaluations of an optimisation driver.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import List, Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from .plan import StepPlan, _dev, _stream
+
+STAGES = ("ygemm", "nl_lin", "solve", "solve_T", "hv_T", "nl_lin_T", "acc")
+
+
+class BatchPlan:
+    def __init__(
+        self,
+        plans: Sequence[StepPlan],
+        states: Sequence[torch.Tensor],
+        adjoints: Optional[Sequence[torch.Tensor]] = None,
+    ):
+        if len(plans) == 0 or len(plans) != len(states):
+            raise _lib.JsdError("one state per plan")
+        if adjoints is not None and len(adjoints) != len(plans):
+            raise _lib.JsdError("one adjoint buffer per plan")
+        for p, s in zip(plans, states):
+            p._check_state(s)
+        self.plans = list(plans)
+        self.states = list(states)
+        self.adjoints = list(adjoints) if adjoints is not None else None
+        n = len(plans)
+        vp = ctypes.c_void_p
+        hp = (vp * n)(*[p._h for p in plans])
+        sp = (vp * n)(*[_dev(s) for s in states])
+        ap = (vp * n)(*[_dev(a) for a in self.adjoints]) if self.adjoints is not None else None
+        h = vp()
+        _lib.check(_lib.load().jsd_batch_create(hp, sp, ap, n, _stream(), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, nsteps: int = 1) -> None:
+        _lib.check(_lib.load().jsd_batch_step(self._h, int(nsteps), _stream()))
+
+    def vjp(self, nsteps: int = 1) -> None:
+        _lib.check(_lib.load().jsd_batch_step_vjp(self._h, int(nsteps), _stream()))
+
+    def info(self) -> dict:
+        g = (ctypes.c_int * len(STAGES))()
+        b = (ctypes.c_int * len(STAGES))()
+        _lib.check(_lib.load().jsd_batch_info(self._h, g, b))
+        return {s: {"grid": g[i], "blocks": b[i]} for i, s in enumerate(STAGES)}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().jsd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _energy_grad(plan: StepPlan, u: torch.Tensor, scale: float) -> torch.Tensor:
+    out = torch.empty_like(u)
+    _lib.check(_lib.load().jsd_energy_grad(plan._h, _dev(u), _dev(out), float(scale), _stream()))
+    return out
+
+
+class BatchGradient:
+    """J_r = E(u_T)/E(u_0) and dJ_r/du_0 for every run of a batch of linearised
+    perturbation plans: one batched forward of nsteps, the energies, one
+    batched adjoint of nsteps (the linearised step's transpose needs no
+    forward state, so nothing is stored or recomputed)."""
+
+    def __init__(self, plans: Sequence[StepPlan]):
+        self.plans = list(plans)
+        dev = torch.device("cuda")
+        self.u = [torch.zeros(p.shape, dtype=torch.complex128, device=dev) for p in self.plans]
+        self.g = [torch.zeros_like(u) for u in self.u]
+        self.batch = BatchPlan(self.plans, self.u, self.g)
+
+    def __call__(self, u0s: Sequence[torch.Tensor], nsteps: int) -> Tuple[List[float], List[torch.Tensor]]:
+        for u, u0 in zip(self.u, u0s):
+            u.copy_(u0)
+        self.batch.step(nsteps)
+        e0 = [p.energy(u0) for p, u0 in zip(self.plans, u0s)]
+        eT = [p.energy(u) for p, u in zip(self.plans, self.u)]
+        J = [a / b for a, b in zip(eT, e0)]
+        for p, u, g, E0 in zip(self.plans, self.u, self.g, e0):
+            g.copy_(_energy_grad(p, u, 1.0 / E0))
+        self.batch.vjp(nsteps)
+        grads = []
+        for p, u0, g, j, E0 in zip(self.plans, u0s, self.g, J, e0):
+            grads.append(g - _energy_grad(p, u0, j / E0))
+        return J, grads
+
+    def close(self) -> None:
+        self.batch.close()
+
+
+def batch_gain_and_grad(
+    plans: Sequence[StepPlan], u0s: Sequence[torch.Tensor], nsteps: int
+) -> Tuple[List[float], List[torch.Tensor]]:
+    bg = BatchGradient(plans)
+    try:
+        return bg(u0s, nsteps)
+    finally:
+        bg.close()

@@ -46,23 +46,50 @@
 enum { GEMM_PLAIN = 0, GEMM_HV = 1, GEMM_HVT = 2, GEMM_ACC = 3 };
 #define GEMM_GK 8  // k-steps per LDS-staged chunk of A fragments
 
+// Logical block of a launch: the single-run kernels pass blockIdx, the
+// batched kernels (one launch for many runs) pass the block within its run.
+struct Blk {
+  int x, y, z;
+  int nx;
+};
+
+struct YgemmArgs {
+  const double* DF;  // D^o pre-tiled (NT row tiles x KS k-steps x 64 lanes)
+  int NT, KS;
+  const double* in0;  // PLAIN: input; HV: H0; HVT: hvbar; ACC: input
+  const double* in1;  // HV: H1; HVT: hgbar
+  const double* in2;  // HV: H2
+  double* out0;       // PLAIN: out; HV: h_v; HVT: Hbar0; ACC: accumulated output
+  double* out1;       // HV: h_g; HVT: Hbar1
+  double* out2;       // HVT: Hbar2
+  double* conv;       // HV: conv lines (2*N complex); HVT: convbar
+  const double* kx;
+  const double* kz;
+  int M, N, Z;
+  long comp_stride;
+};
+
 template <int MODE>
-__global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, int NT, int KS,
-                                               const double* __restrict__ in0,  // PLAIN: input; HV: H0
-                                               const double* __restrict__ in1,  // HV: H1
-                                               const double* __restrict__ in2,  // HV: H2
-                                               double* __restrict__ out0,       // PLAIN: out; HV: h_v
-                                               double* __restrict__ out1,       // HV: h_g
-                                               double* __restrict__ out2,       // HVT: third output
-                                               double* __restrict__ conv,       // HV: conv lines (2*N complex)
-                                               const double* __restrict__ kx, const double* __restrict__ kz, int M,
-                                               int N, int Z, long comp_stride) {
+__device__ void ygemm_body(const YgemmArgs& G, Blk blk) {
+  const double* __restrict__ DF = G.DF;
+  const int NT = G.NT, KS = G.KS;
+  const double* __restrict__ in0 = G.in0;
+  const double* __restrict__ in1 = G.in1;
+  const double* __restrict__ in2 = G.in2;
+  double* __restrict__ out0 = G.out0;
+  double* __restrict__ out1 = G.out1;
+  double* __restrict__ out2 = G.out2;
+  double* __restrict__ conv = G.conv;
+  const double* __restrict__ kx = G.kx;
+  const double* __restrict__ kz = G.kz;
+  const int N = G.N, Z = G.Z;
+  const long comp_stride = G.comp_stride;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int i = blockIdx.x;
-  const int comp = blockIdx.z;
+  const int i = blk.x;
+  const int comp = blk.z;
   const int C2 = 2 * Z;
-  const int col = blockIdx.y * 64 + wave * 16 + (lane & 15);
+  const int col = blk.y * 64 + wave * 16 + (lane & 15);
   const bool colok = col < C2;
   const int kr = lane >> 4;
   const size_t rowbase = (size_t)comp * comp_stride + (size_t)i * N * C2;
@@ -186,6 +213,11 @@ __global__ __launch_bounds__(256) void k_ygemm(const double* __restrict__ DF, in
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ygemm(YgemmArgs G) {
+  ygemm_body<MODE>(G, Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x});
+}
+
 // ===========================================================================
 // FFT passes (nonlinear path)
 // ===========================================================================
@@ -204,6 +236,14 @@ __device__ __forceinline__ int slot_col(int c, int q0, int Z, int KZ, bool* ok) 
 __device__ __forceinline__ int slot_partner(int c, int q0) {
   const int q = q0 + (c % QB);
   return q == 0 ? c : (c ^ QB);
+}
+
+// XCD-aware decode of a 1-D grid of 8*ceil(total/8) blocks: blocks are
+// dispatched round-robin over the 8 XCDs, so logical index g =
+// (bid % 8) * per + bid / 8 puts consecutive g on one XCD (one L2).
+__device__ __forceinline__ int xcd_logical() {
+  const int per = (int)(gridDim.x / 8);
+  return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
 enum { OP_ID = 0, OP_DX = 1, OP_DZ = 2 };
@@ -226,13 +266,18 @@ struct XInvSet {
   int opB[JSD_MAXTR];
 };
 
-__global__ __launch_bounds__(256) void k_fft_x_inv(XInvSet set, FftPlan PX, const double* __restrict__ kx,
-                                                   const double* __restrict__ kz, int M, int N, int Z, double s_inv) {
+__global__ __launch_bounds__(256) void k_fft_x_inv(XInvSet set, int ntr, int nqb, FftPlan PX,
+                                                   const double* __restrict__ kx, const double* __restrict__ kz, int M,
+                                                   int N, int Z, int ZT, double s_inv) {
   extern __shared__ double2 lds[];
   double2* buf = lds;  // [p][slot]
-  const int q0 = blockIdx.x * QB;
-  const int l = blockIdx.y;
-  const int tr = blockIdx.z;
+  // logical order (l, qb, tr), tr fastest: the transforms reading the same
+  // velocity component run together on one XCD
+  const int g = xcd_logical();
+  if (g >= ntr * nqb * N) return;
+  const int tr = g % ntr;
+  const int q0 = ((g / ntr) % nqb) * QB;
+  const int l = g / (ntr * nqb);
   const double2* __restrict__ A = set.A[tr];
   const double2* __restrict__ B = set.B[tr];
   const int opA = set.opA[tr], opB = set.opB[tr];
@@ -271,7 +316,7 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(XInvSet set, FftPlan PX, cons
     const int x = w / NSLOT;
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
-    if (ok) T[((size_t)x * N + l) * Z + j] = buf[x * NSLOT + c];
+    if (ok) T[((size_t)x * N + l) * ZT + j] = buf[x * NSLOT + c];
   }
 }
 
@@ -351,7 +396,7 @@ __device__ __forceinline__ void zmid_products(const double2* row, double U, doub
 #define ZMID_ITEMS 8
 __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T, long tstride, int nt,
                                                    double2* __restrict__ V, long vstride, FftPlan PZ, int N, int Z,
-                                                   int prod, double clin, const double* __restrict__ Ub,
+                                                   int ZT, int prod, double clin, const double* __restrict__ Ub,
                                                    const double* __restrict__ dUb, int YB) {
   extern __shared__ double2 lds[];
   const int P = PZ.p;
@@ -370,7 +415,7 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
     const int yb = r / nt, t = r - yb * nt;
     if (yb >= nyb) continue;
     const int n = (j <= KZ) ? j : P - (Z - j);
-    lds[PZ.iperm[n] * R + r] = T[t * tstride + ((size_t)x * N + l0 + yb) * Z + j];
+    lds[PZ.iperm[n] * R + r] = T[t * tstride + ((size_t)x * N + l0 + yb) * ZT + j];
   }
   __syncthreads();
   fft_lds(lds, R, R, 1, PZ, +1.0);
@@ -407,7 +452,7 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
     const int yb = r / 2, o = r % 2;
     if (yb >= nyb) continue;
     const int n = (j <= KZ) ? j : P - (Z - j);
-    V[o * vstride + ((size_t)x * N + l0 + yb) * Z + j] = lds[n * R2 + r];
+    V[o * vstride + ((size_t)x * N + l0 + yb) * ZT + j] = lds[n * R2 + r];
   }
 }
 
@@ -416,7 +461,7 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
 // H0, H1; t=1 writes H2.
 __global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V, long vstride,
                                                    double2* __restrict__ H, long hstride, FftPlan PX, int M, int N,
-                                                   int Z, double s_fwd) {
+                                                   int Z, int ZT, double s_fwd) {
   extern __shared__ double2 lds[];
   double2* buf = lds;
   const int q0 = blockIdx.x * QB;
@@ -431,7 +476,7 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
     const int xi = PX.perm[pos];
-    buf[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * Z + j] : make_double2(0.0, 0.0);
+    buf[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * ZT + j] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   fft_lds(buf, NSLOT, NSLOT, 1, PX, -1.0);
@@ -476,14 +521,16 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V
 // the supported configurations and the run-time kernels otherwise.
 
 template <class C>
-__global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, const double2* __restrict__ tw,
+__global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, int ntr, int nqb, const double2* __restrict__ tw,
                                                      const double* __restrict__ kx, const double* __restrict__ kz,
-                                                     int M, int N, int Z, double s_inv) {
+                                                     int M, int N, int Z, int ZT, double s_inv) {
   extern __shared__ double2 lds[];
   constexpr int P = C::P;
-  const int q0 = blockIdx.x * QB;
-  const int l = blockIdx.y;
-  const int tr = blockIdx.z;
+  const int g = xcd_logical();
+  if (g >= ntr * nqb * N) return;
+  const int tr = g % ntr;
+  const int q0 = ((g / ntr) % nqb) * QB;
+  const int l = g / (ntr * nqb);
   const double2* __restrict__ A = set.A[tr];
   const double2* __restrict__ B = set.B[tr];
   const int opA = set.opA[tr], opB = set.opB[tr];
@@ -521,7 +568,7 @@ __global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, const double2*
     const int x = w / NSLOT;
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
-    if (ok) T[((size_t)x * N + l) * Z + j] = lds[x * NSLOT + c];
+    if (ok) T[((size_t)x * N + l) * ZT + j] = lds[x * NSLOT + c];
   }
 }
 
@@ -533,7 +580,7 @@ template <class C, int NT, int PROD>
 __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double2* __restrict__ T, long tstride,
                                                                     double2* __restrict__ V, long vstride,
                                                                     const double2* __restrict__ tw, int N, int Z,
-                                                                    double clin, const double* __restrict__ Ub,
+                                                                    int ZT, double clin, const double* __restrict__ Ub,
                                                                     const double* __restrict__ dUb) {
   extern __shared__ double2 lds[];
   constexpr int P = C::P;
@@ -542,7 +589,7 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
   const int x = blockIdx.x;
   const int l = blockIdx.y;
   const int KZ = Z / 2;
-  const size_t line = ((size_t)x * N + l) * Z;
+  const size_t line = ((size_t)x * N + l) * ZT;
   for (int w = threadIdx.x; w < P * NT; w += NTH) lds[w] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int j = threadIdx.x; j < Z; j += NTH) {
@@ -586,7 +633,7 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
 template <class C>
 __global__ __launch_bounds__(256) void k_fft_x_fwd_t(const double2* __restrict__ V, long vstride,
                                                      double2* __restrict__ H, long hstride,
-                                                     const double2* __restrict__ tw, int M, int N, int Z,
+                                                     const double2* __restrict__ tw, int M, int N, int Z, int ZT,
                                                      double s_fwd) {
   extern __shared__ double2 lds[];
   constexpr int P = C::P;
@@ -601,7 +648,7 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd_t(const double2* __restrict__
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
     const int xi = dit_perm<C, C::NS>(pos);
-    lds[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * Z + j] : make_double2(0.0, 0.0);
+    lds[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * ZT + j] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   fft_lds_ct<C, NSLOT, NSLOT, 1, 256, -1>(lds, tw);
@@ -692,12 +739,30 @@ static bool visit_z(const FftPlan& f, F&& fn) {
 // G2 = -U ikx w.  Dual (navier_stokes_perturbation_dual.py:214-294 with
 // linearise): G0 = -U ikx v0 + 0.25 U div v, G1 = -U ikx v1 + U' v0,
 // G2 = -U ikx v2.
-__global__ __launch_bounds__(256) void k_nl_lin(const double2* __restrict__ U0, long cs, const double2* __restrict__ DV,
-                                                double2* __restrict__ H, const double* __restrict__ kx,
-                                                const double* __restrict__ kz, const double* __restrict__ Ub,
-                                                const double* __restrict__ dUb, int M, int N, int Z, int dual) {
+struct NlLinArgs {
+  const double2* U0;  // state (3 components)
+  long cs;
+  const double2* DV;  // D v
+  double2* H;         // out: H (3 components, stored convention)
+  const double* kx;
+  const double* kz;
+  const double* Ub;
+  const double* dUb;
+  int M, N, Z, dual;
+};
+
+__device__ __forceinline__ void nl_lin_body(const NlLinArgs& A, Blk blk) {
+  const double2* __restrict__ U0 = A.U0;
+  const long cs = A.cs;
+  const double2* __restrict__ DV = A.DV;
+  double2* __restrict__ H = A.H;
+  const double* __restrict__ kx = A.kx;
+  const double* __restrict__ kz = A.kz;
+  const double* __restrict__ Ub = A.Ub;
+  const double* __restrict__ dUb = A.dUb;
+  const int M = A.M, N = A.N, Z = A.Z, dual = A.dual;
   const size_t tot = (size_t)M * N * Z;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t e = (size_t)blk.x * 256 + threadIdx.x;
   if (e >= tot) return;
   const int j = e % Z;
   const int l = (e / Z) % N;
@@ -736,6 +801,10 @@ __global__ __launch_bounds__(256) void k_nl_lin(const double2* __restrict__ U0, 
   H[e] = g0;
   H[cs + e] = g1;
   H[2 * cs + e] = g2;
+}
+
+__global__ __launch_bounds__(256) void k_nl_lin(NlLinArgs A) {
+  nl_lin_body(A, Blk{(int)blockIdx.x, 0, 0, (int)gridDim.x});
 }
 
 // ===========================================================================
@@ -881,6 +950,7 @@ struct DevBuf {
 struct jsd_plan {
   jsd_plan_desc d;
   int M, N, Z, K, KZ, Ka, Kb, NT, KS;
+  int ZT;      // row pitch of the x-z intermediates T, V (Z rounded up to 8: 128-byte rows)
   size_t fs;   // complex elements per field
   double s_inv, s_fwd, dpdx_scale;
   std::vector<void*> allocs;
@@ -1085,6 +1155,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   p->Z = Z;
   p->K = M / 2;
   p->KZ = Z / 2;
+  p->ZT = (Z + 7) & ~7;
   p->Ka = p->K + 1;
   p->Kb = p->KZ + 1;
   p->NT = (N + 15) / 16;
@@ -1253,7 +1324,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   const bool nonlin = (d->kind == JSD_KIND_NS) || !d->linearise;
   if (nonlin) {
     const int ntr = (d->kind == JSD_KIND_DUAL) ? 12 : 6;
-    const size_t ts = (size_t)d->ppx * N * Z;
+    const size_t ts = (size_t)d->ppx * N * p->ZT;
     TRY(dalloc(p, (void**)&p->T, ntr * ts * sizeof(double2)));
     TRY(dalloc(p, (void**)&p->V, 2 * ts * sizeof(double2)));
   } else {
@@ -1274,26 +1345,64 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
 }
 
 // ---------------------------------------------------------------------------
+static YgemmArgs ygemm_args(const jsd_plan* p, const double* DF, const void* in0, const void* in1, const void* in2,
+                            void* out0, void* out1, void* out2, void* conv, long comp_stride) {
+  YgemmArgs G;
+  G.DF = DF;
+  G.NT = p->NT;
+  G.KS = p->KS;
+  G.in0 = (const double*)in0;
+  G.in1 = (const double*)in1;
+  G.in2 = (const double*)in2;
+  G.out0 = (double*)out0;
+  G.out1 = (double*)out1;
+  G.out2 = (double*)out2;
+  G.conv = (double*)conv;
+  G.kx = p->kx;
+  G.kz = p->kz;
+  G.M = p->M;
+  G.N = p->N;
+  G.Z = p->Z;
+  G.comp_stride = comp_stride;
+  return G;
+}
+static dim3 ygemm_grid(const jsd_plan* p, int ncomp) { return dim3(p->M, (2 * p->Z + 63) / 64, ncomp); }
+
 static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int ncomp, int order, hipStream_t st) {
-  dim3 grid(p->M, (2 * p->Z + 63) / 64, ncomp);
   prof_begin(p, 0, st);
-  hipLaunchKernelGGL(k_ygemm<GEMM_PLAIN>, grid, dim3(256), 0, st, order == 2 ? p->DF2 : p->DF1, p->NT, p->KS,
-                     (const double*)in, nullptr, nullptr, (double*)out, nullptr, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
-                     p->Z, (long)(2 * p->fs));
+  hipLaunchKernelGGL(k_ygemm<GEMM_PLAIN>, ygemm_grid(p, ncomp), dim3(256), 0, st,
+                     ygemm_args(p, order == 2 ? p->DF2 : p->DF1, in, nullptr, nullptr, out, nullptr, nullptr, nullptr,
+                                (long)(2 * p->fs)));
   prof_end(p, 0, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
 }
 
 static int launch_hv(jsd_plan* p, const double2* H, double2* hv, double2* hg, double2* conv, hipStream_t st) {
-  dim3 grid(p->M, (2 * p->Z + 63) / 64, 1);
   const double* h = (const double*)H;
   prof_begin(p, 4, st);
-  hipLaunchKernelGGL(k_ygemm<GEMM_HV>, grid, dim3(256), 0, st, p->DF1, p->NT, p->KS, h, h + 2 * p->fs,
-                     h + 4 * p->fs, (double*)hv, (double*)hg, nullptr, (double*)conv, p->kx, p->kz, p->M, p->N, p->Z, 0L);
+  hipLaunchKernelGGL(k_ygemm<GEMM_HV>, ygemm_grid(p, 1), dim3(256), 0, st,
+                     ygemm_args(p, p->DF1, h, h + 2 * p->fs, h + 4 * p->fs, hv, hg, nullptr, conv, 0L));
   prof_end(p, 4, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
+}
+
+static NlLinArgs nl_lin_args(const jsd_plan* p, const double2* U) {
+  NlLinArgs A;
+  A.U0 = U;
+  A.cs = (long)p->fs;
+  A.DV = p->DU;
+  A.H = p->H;
+  A.kx = p->kx;
+  A.kz = p->kz;
+  A.Ub = p->Ub;
+  A.dUb = p->dUb;
+  A.M = p->M;
+  A.N = p->N;
+  A.Z = p->Z;
+  A.dual = p->d.kind == JSD_KIND_DUAL ? 1 : 0;
+  return A;
 }
 
 // H (3 comps, stored convention) from the state U (and the forward state for a nonlinear dual)
@@ -1307,8 +1416,7 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     if (rc) return rc;
     const size_t tot = fs;
     prof_begin(p, 6, st);
-    hipLaunchKernelGGL(k_nl_lin, dim3((tot + 255) / 256), dim3(256), 0, st, U, (long)fs, p->DU, p->H, p->kx,
-                       p->kz, p->Ub, p->dUb, M, N, Z, d.kind == JSD_KIND_DUAL ? 1 : 0);
+    hipLaunchKernelGGL(k_nl_lin, dim3((tot + 255) / 256), dim3(256), 0, st, nl_lin_args(p, U));
     prof_end(p, 6, st);
     HIPCHK(hipGetLastError());
     return JSD_OK;
@@ -1316,7 +1424,8 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   int rc = launch_ygemm_plain(p, U, p->DU, 3, 1, st);
   if (rc) return rc;
   const int nqb = (p->KZ + 1 + QB - 1) / QB;
-  const size_t ts = (size_t)d.ppx * N * Z;
+  const int ZT = p->ZT;
+  const size_t ts = (size_t)d.ppx * N * ZT;
   const size_t lds_x = (size_t)d.ppx * NSLOT * sizeof(double2);
   int prod = PROD_NS;
   double clin = 1.0;
@@ -1354,12 +1463,13 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   for (int t = 0; t < ntr; ++t) set.T[t] = p->T + t * ts;
   const bool ct = !p->generic_fft;
   prof_begin(p, 1, st);
+  const dim3 grid_xi(8 * ((ntr * nqb * N + 7) / 8));
   const bool xi_ct = ct && visit_x(p->fpx, [&](auto c) {
-    hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, dim3(nqb, N, ntr), dim3(256), lds_x, st, set, p->fpx.tw, p->kx,
-                       p->kz, M, N, Z, p->s_inv);
+    hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx.tw, p->kx,
+                       p->kz, M, N, Z, ZT, p->s_inv);
   });
   if (!xi_ct)
-    hipLaunchKernelGGL(k_fft_x_inv, dim3(nqb, N, ntr), dim3(256), lds_x, st, set, p->fpx, p->kx, p->kz, M, N, Z,
+    hipLaunchKernelGGL(k_fft_x_inv, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx, p->kx, p->kz, M, N, Z, ZT,
                        p->s_inv);
   prof_end(p, 1, st);
   HIPCHK(hipGetLastError());
@@ -1370,13 +1480,13 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     const dim3 grid(d.ppx, N), block(zmid_threads(C::P));
     if (prod == PROD_NS)
       hipLaunchKernelGGL((k_fft_z_mid_t<C, 6, PROD_NS>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
-                         p->fpz.tw, N, Z, clin, p->Ub, p->dUb);
+                         p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
     else if (prod == PROD_PERT)
       hipLaunchKernelGGL((k_fft_z_mid_t<C, 6, PROD_PERT>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
-                         p->fpz.tw, N, Z, clin, p->Ub, p->dUb);
+                         p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
     else
       hipLaunchKernelGGL((k_fft_z_mid_t<C, 12, PROD_DUAL>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
-                         p->fpz.tw, N, Z, clin, p->Ub, p->dUb);
+                         p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
   });
   if (!z_ct) {
     // y planes per z-pass workgroup: LDS <= 80 KB and <= ZMID_ITEMS items per thread
@@ -1384,41 +1494,48 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     YB = std::max(1, std::min(YB, std::min(4, (ZMID_ITEMS * 256) / d.ppz)));
     const size_t lds_z = (size_t)d.ppz * ntr * YB * sizeof(double2);
     hipLaunchKernelGGL(k_fft_z_mid, dim3(d.ppx, (N + YB - 1) / YB), dim3(256), lds_z, st, p->T, (long)ts, ntr, p->V,
-                       (long)ts, p->fpz, N, Z, prod, clin, p->Ub, p->dUb, YB);
+                       (long)ts, p->fpz, N, Z, ZT, prod, clin, p->Ub, p->dUb, YB);
   }
   prof_end(p, 2, st);
   HIPCHK(hipGetLastError());
   prof_begin(p, 3, st);
   const bool xf_ct = ct && visit_x(p->fpx, [&](auto c) {
     hipLaunchKernelGGL(k_fft_x_fwd_t<decltype(c)>, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H,
-                       (long)fs, p->fpx.tw, M, N, Z, p->s_fwd);
+                       (long)fs, p->fpx.tw, M, N, Z, ZT, p->s_fwd);
   });
   if (!xf_ct)
     hipLaunchKernelGGL(k_fft_x_fwd, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx,
-                       M, N, Z, p->s_fwd);
+                       M, N, Z, ZT, p->s_fwd);
   prof_end(p, 3, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
 }
 
-static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st) {
-  static const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
-  static const double xi[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
-  static const double alp[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
-  int rc = nonlinear_H(p, U, UF, st);
-  if (rc) return rc;
+static const double RK_GAM[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double RK_XI[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+static const double RK_ALP[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
+
+// Buffers of the old-term carry between substeps: the current substep writes
+// HV/HG/CONV and reads HVo/HGo/CONVo (the previous substep's output).
+struct Roles {
+  double2 *HV, *HG, *CONV, *HVo, *HGo, *CONVo;
+};
+static Roles roles_of(const jsd_plan* p) { return Roles{p->HV, p->HG, p->CONV, p->HVo, p->HGo, p->CONVo}; }
+static Roles roles_swapped(const Roles& r) { return Roles{r.HVo, r.HGo, r.CONVo, r.HV, r.HG, r.CONV}; }
+
+static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, const Roles& r) {
   const size_t nn = (size_t)p->N * p->N;
   const size_t nk = (size_t)p->Ka * p->Kb;
   SolveArgs S;
   S.U = U;
   S.cs = (long)p->fs;
   S.H = p->H;
-  S.HV = p->HV;
-  S.HG = p->HG;
-  S.HVo = p->HVo;
-  S.HGo = p->HGo;
-  S.CONV = p->CONV;
-  S.CONVo = p->CONVo;
+  S.HV = r.HV;
+  S.HG = r.HG;
+  S.HVo = r.HVo;
+  S.HGo = r.HGo;
+  S.CONV = r.CONV;
+  S.CONVo = r.CONVo;
   S.DEO = p->DEO;
   S.DOE = p->DOE;
   S.D2E = p->D2E;
@@ -1443,16 +1560,22 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   S.nE = p->nE;
   S.nO = p->nO;
   const double dt = p->d.dt;
-  S.dt_gamma = dt * gam[s];
-  S.dt_xi = dt * xi[s];
+  S.dt_gamma = dt * RK_GAM[s];
+  S.dt_xi = dt * RK_XI[s];
   // alpha_s dt (-k^2)/Re enters as diag_alpha * (-k^2) with diag_alpha = alpha_s*dt/Re;
   // the tables were built with dt_tab/Re_tab = dt/Re (the dual flips both signs)
-  S.diag_alpha = alp[s] * fabs(dt) / fabs(p->d.re_tau);
+  S.diag_alpha = RK_ALP[s] * fabs(dt) / fabs(p->d.re_tau);
   S.dpdx = dPdx * p->dpdx_scale;
   S.use_old = s > 0;
+  return S;
+}
+
+static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st) {
+  int rc = nonlinear_H(p, U, UF, st);
+  if (rc) return rc;
   const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
   prof_begin(p, 5, st);
-  hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, S);
+  hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, solve_args(p, U, s, dPdx, roles_of(p)));
   prof_end(p, 5, st);
   HIPCHK(hipGetLastError());
   std::swap(p->HV, p->HVo);
@@ -1548,21 +1671,34 @@ extern "C" int jsd_profile(jsd_plan* p, int enable, double* ms, long* count) {
 // ---------------------------------------------------------------------------
 // discrete adjoint of the linearised perturbation step
 // ---------------------------------------------------------------------------
-static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
-  static const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
-  static const double xi[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
-  static const double alp[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
+static NlLinTArgs nl_lin_T_args(const jsd_plan* p, double2* UB) {
+  NlLinTArgs A;
+  A.HB = p->H;
+  A.cs = (long)p->fs;
+  A.UB = UB;
+  A.DVB = p->DU;
+  A.kx = p->kx;
+  A.kz = p->kz;
+  A.Ub = p->Ub;
+  A.dUb = p->dUb;
+  A.M = p->M;
+  A.N = p->N;
+  A.Z = p->Z;
+  return A;
+}
+
+static SolveTArgs solve_T_args(const jsd_plan* p, double2* UB, int s, bool has_carry, const Roles& r) {
   const size_t nn = (size_t)p->N * p->N;
   const size_t nk = (size_t)p->Ka * p->Kb;
   SolveTArgs S;
   S.UB = UB;
   S.cs = (long)p->fs;
-  S.HVB = p->HV;
-  S.HGB = p->HG;
-  S.HVO = p->HVo;
-  S.HGO = p->HGo;
-  S.CB = p->CONV;
-  S.CBO = p->CONVo;
+  S.HVB = r.HV;
+  S.HGB = r.HG;
+  S.HVO = r.HVo;
+  S.HGO = r.HGo;
+  S.CB = r.CONV;
+  S.CBO = r.CONVo;
   S.D_R = p->D_R;
   S.D2_R = p->D2_R;
   S.RB_R = p->RB_R + s * nn;
@@ -1580,30 +1716,34 @@ static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipS
   S.Ka = p->Ka;
   S.Kb = p->Kb;
   const double dt = p->d.dt;
-  S.dt_gamma = dt * gam[s];
-  S.dt_xi = dt * xi[s];
-  S.diag_alpha = alp[s] * fabs(dt) / fabs(p->d.re_tau);
+  S.dt_gamma = dt * RK_GAM[s];
+  S.dt_xi = dt * RK_XI[s];
+  S.diag_alpha = RK_ALP[s] * fabs(dt) / fabs(p->d.re_tau);
   S.use_old = s > 0;
   S.has_carry = has_carry ? 1 : 0;
-  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
-  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, S);
-  HIPCHK(hipGetLastError());
-  // Hbar from (hvbar, hgbar, convbar): transposed h_v / h_g assembly
-  dim3 grid(p->M, (2 * p->Z + 63) / 64, 1);
+  return S;
+}
+// Hbar from (hvbar, hgbar, convbar): transposed h_v / h_g assembly
+static YgemmArgs hvt_args(const jsd_plan* p, const Roles& r) {
   double* hb = (double*)p->H;
-  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, grid, dim3(256), 0, st, p->DFT1, p->NT, p->KS, (const double*)p->HV,
-                     (const double*)p->HG, nullptr, hb, hb + 2 * p->fs, hb + 4 * p->fs, (double*)p->CONV, p->kx,
-                     p->kz, p->M, p->N, p->Z, 0L);
+  return ygemm_args(p, p->DFT1, r.HV, r.HG, nullptr, hb, hb + 2 * p->fs, hb + 4 * p->fs, r.CONV, 0L);
+}
+// vbar += D^T Dvbar
+static YgemmArgs acc_args(const jsd_plan* p, double2* UB) {
+  return ygemm_args(p, p->DFT1, p->DU, nullptr, nullptr, UB + p->fs, nullptr, nullptr, nullptr, 0L);
+}
+
+static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
+  const Roles r = roles_of(p);
+  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));
   HIPCHK(hipGetLastError());
   // state adjoint += transposed linearised product term; Dvbar -> DU[0]
-  const size_t tot = p->fs;
-  hipLaunchKernelGGL(k_nl_lin_T, dim3((tot + 255) / 256), dim3(256), 0, st, p->H, (long)p->fs, UB, p->DU, p->kx,
-                     p->kz, p->Ub, p->dUb, p->M, p->N, p->Z);
+  hipLaunchKernelGGL(k_nl_lin_T, dim3((p->fs + 255) / 256), dim3(256), 0, st, nl_lin_T_args(p, UB));
   HIPCHK(hipGetLastError());
-  // vbar += D^T Dvbar
-  hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, grid, dim3(256), 0, st, p->DFT1, p->NT, p->KS, (const double*)p->DU,
-                     nullptr, nullptr, (double*)(UB + p->fs), nullptr, nullptr, nullptr, p->kx, p->kz, p->M, p->N,
-                     p->Z, 0L);
+  hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, ygemm_grid(p, 1), dim3(256), 0, st, acc_args(p, UB));
   HIPCHK(hipGetLastError());
   // carries of this substep's old-term usage feed the previous substep
   std::swap(p->HV, p->HVo);
@@ -1633,5 +1773,390 @@ extern "C" int jsd_energy_grad(jsd_plan* p, const void* vel_hat, void* out, doub
   hipLaunchKernelGGL(k_energy_grad, dim3((tot + 255) / 256), dim3(256), 0, (hipStream_t)stream,
                      (const double2*)vel_hat, (long)p->fs, (double2*)out, p->wy, p->M, p->N, p->Z, c);
   HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+// ===========================================================================
+// batches of independent linearised runs
+// ===========================================================================
+// Every stage kernel of the linearised step has a body that takes its
+// arguments and logical block explicitly; k_batched walks the concatenated
+// per-run grids (run r owns logical blocks [first_r, first_{r+1})) with a
+// persistent grid of any size.
+
+struct OpYgemmPlain {
+  typedef YgemmArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_PLAIN>(a, k); }
+};
+struct OpYgemmHVT {
+  typedef YgemmArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_HVT>(a, k); }
+};
+struct OpYgemmAcc {
+  typedef YgemmArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_ACC>(a, k); }
+};
+struct OpNlLin {
+  typedef NlLinArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { nl_lin_body(a, k); }
+};
+struct OpNlLinT {
+  typedef NlLinTArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { nl_lin_T_body(a, k); }
+};
+struct OpSolve {
+  typedef SolveArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { solve_body(a, k.x); }
+};
+struct OpSolveT {
+  typedef SolveTArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { solve_T_body(a, k.x); }
+};
+
+// runs[r] = (nx, ny, nz, first logical block of run r), ordered by first
+template <class Op>
+__global__ __launch_bounds__(256) void k_batched(const typename Op::Args* __restrict__ args,
+                                                 const int4* __restrict__ runs, int nrun, int total) {
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    int lo = 0, hi = nrun - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (runs[mid].w <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    const int4 R = runs[lo];
+    const int lb = b - R.w;
+    Op::run(args[lo], Blk{lb % R.x, (lb / R.x) % R.y, lb / (R.x * R.y), R.x});
+    __syncthreads();  // the next logical block reuses the LDS
+  }
+}
+
+enum { BS_YG = 0, BS_NL, BS_SOLVE, BS_SOLVET, BS_HVT, BS_NLT, BS_ACC, BS_NKIND };
+
+struct BatchStage {
+  void* args = nullptr;
+  int4* runs = nullptr;
+  int total = 0;
+};
+
+struct jsd_batch {
+  int nrun = 0;
+  bool vjp = false;
+  std::vector<jsd_plan*> plans;
+  std::vector<double2*> state, adj;
+  std::vector<void*> allocs;
+  BatchStage yg, nl, solve[3];          // forward: y-GEMM (D v), linearised term, solve per substep
+  BatchStage solveT[3], hvt[3], nlT, acc;  // adjoint, indexed by call order c (substep 2 - c)
+  int cap[BS_NKIND];
+  int total[BS_NKIND];
+  hipStream_t cs = nullptr;
+  hipGraphExec_t xf = nullptr, xb = nullptr;
+  hipGraph_t gf = nullptr, gb = nullptr;
+};
+
+static int bdalloc(jsd_batch* b, void** ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) JSD_FAIL(JSD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  b->allocs.push_back(*ptr);
+  return JSD_OK;
+}
+
+template <class A>
+static int make_stage(jsd_batch* b, BatchStage& S, const std::vector<A>& args, const std::vector<dim3>& grids) {
+  std::vector<int4> runs(args.size());
+  int first = 0;
+  for (size_t r = 0; r < args.size(); ++r) {
+    runs[r] = make_int4((int)grids[r].x, (int)grids[r].y, (int)grids[r].z, first);
+    first += (int)(grids[r].x * grids[r].y * grids[r].z);
+  }
+  S.total = first;
+  int rc = bdalloc(b, &S.args, args.size() * sizeof(A));
+  if (rc) return rc;
+  rc = bdalloc(b, (void**)&S.runs, runs.size() * sizeof(int4));
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(S.args, args.data(), args.size() * sizeof(A), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(S.runs, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
+  return JSD_OK;
+}
+
+template <class Op>
+static int launch_stage(const jsd_batch* b, const BatchStage& S, int cap, hipStream_t st) {
+  const int grid = std::max(1, std::min(S.total, cap));
+  hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(256), 0, st, (const typename Op::Args*)S.args, S.runs,
+                     b->nrun, S.total);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+static int batch_fwd_step(const jsd_batch* b, hipStream_t st) {
+  for (int s = 0; s < 3; ++s) {
+    int rc = launch_stage<OpYgemmPlain>(b, b->yg, b->cap[BS_YG], st);
+    if (!rc) rc = launch_stage<OpNlLin>(b, b->nl, b->cap[BS_NL], st);
+    if (!rc) rc = launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
+    if (rc) return rc;
+  }
+  return JSD_OK;
+}
+
+static int batch_bwd_step(const jsd_batch* b, hipStream_t st) {
+  for (int c = 0; c < 3; ++c) {
+    int rc = launch_stage<OpSolveT>(b, b->solveT[c], b->cap[BS_SOLVET], st);
+    if (!rc) rc = launch_stage<OpYgemmHVT>(b, b->hvt[c], b->cap[BS_HVT], st);
+    if (!rc) rc = launch_stage<OpNlLinT>(b, b->nlT, b->cap[BS_NLT], st);
+    if (!rc) rc = launch_stage<OpYgemmAcc>(b, b->acc, b->cap[BS_ACC], st);
+    if (rc) return rc;
+  }
+  return JSD_OK;
+}
+
+// time one stage kernel (all its substep variants) with a candidate grid size
+template <class Op>
+static float time_stage(const jsd_batch* b, const BatchStage* S, int nS, int cap, hipStream_t st) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return 1e30f;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return 1e30f;
+  }
+  float best = 1e30f;
+  for (int rep = 0; rep < 3; ++rep) {
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < nS; ++i)
+      if (launch_stage<Op>(b, S[i], cap, st)) break;
+    (void)hipEventRecord(e1, st);
+    float ms = 1e30f;
+    if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && rep > 0)
+      best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+template <class Op>
+static void tune_stage(jsd_batch* b, int kind, const BatchStage* S, int nS, hipStream_t st) {
+  // candidates: one block per logical block, or persistent grids of 1..8 blocks per CU
+  const int cands[5] = {S[0].total, 256 * 8, 256 * 4, 256 * 2, 256};
+  int best = S[0].total;
+  float tbest = 1e30f;
+  for (int c : cands) {
+    if (c > S[0].total) continue;
+    const float t = time_stage<Op>(b, S, nS, c, st);
+    if (t < tbest * 0.97f) {
+      tbest = t;
+      best = c;
+    }
+  }
+  b->cap[kind] = best;
+}
+
+extern "C" int jsd_batch_destroy(jsd_batch* b) {
+  if (!b) return JSD_OK;
+  if (b->xf) (void)hipGraphExecDestroy(b->xf);
+  if (b->xb) (void)hipGraphExecDestroy(b->xb);
+  if (b->gf) (void)hipGraphDestroy(b->gf);
+  if (b->gb) (void)hipGraphDestroy(b->gb);
+  if (b->cs) (void)hipStreamDestroy(b->cs);
+  for (void* a : b->allocs) (void)hipFree(a);
+  delete b;
+  return JSD_OK;
+}
+
+static int capture(jsd_batch* b, bool fwd, hipGraph_t* g, hipGraphExec_t* x) {
+  HIPCHK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+  const int rc = fwd ? batch_fwd_step(b, b->cs) : batch_bwd_step(b, b->cs);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->cs, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) JSD_FAIL(JSD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  *g = graph;
+  HIPCHK(hipGraphInstantiate(x, graph, nullptr, nullptr, 0));
+  return JSD_OK;
+}
+
+extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, void* const* adjoints, int nrun,
+                                void* stream, jsd_batch** out) {
+  if (!plans || !states || nrun <= 0 || !out) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  *out = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  jsd_batch* b = new jsd_batch();
+  b->nrun = nrun;
+  b->vjp = adjoints != nullptr;
+  for (int r = 0; r < nrun; ++r) {
+    jsd_plan* p = plans[r];
+    if (!p || !states[r] || (b->vjp && !adjoints[r])) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_ARG, "run %d: missing plan or buffer", r);
+    }
+    if (!p->d.linearise || p->d.kind == JSD_KIND_NS) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_STATE, "run %d: batches hold linearised perturbation or dual plans only", r);
+    }
+    if (b->vjp && (p->d.kind != JSD_KIND_PERTURBATION || !p->LI_R)) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_STATE, "run %d: the adjoint is built for linearised perturbation plans", r);
+    }
+    b->plans.push_back(p);
+    b->state.push_back((double2*)states[r]);
+    b->adj.push_back(b->vjp ? (double2*)adjoints[r] : nullptr);
+  }
+#define BTRY(x)               \
+  do {                        \
+    int _rc = (x);            \
+    if (_rc) {                \
+      jsd_batch_destroy(b);   \
+      return _rc;             \
+    }                         \
+  } while (0)
+  // forward stages; the carry buffers alternate with the substep (any
+  // assignment is valid at step boundaries: substep 0 reads no old terms)
+  {
+    std::vector<YgemmArgs> ya;
+    std::vector<NlLinArgs> na;
+    std::vector<dim3> yg, ng, sg;
+    for (int r = 0; r < nrun; ++r) {
+      jsd_plan* p = b->plans[r];
+      double2* U = b->state[r];
+      ya.push_back(ygemm_args(p, p->DF1, U + p->fs, nullptr, nullptr, p->DU, nullptr, nullptr, nullptr,
+                              (long)(2 * p->fs)));
+      yg.push_back(ygemm_grid(p, 1));
+      na.push_back(nl_lin_args(p, U));
+      ng.push_back(dim3((unsigned)((p->fs + 255) / 256)));
+      sg.push_back(dim3(p->Ka * p->Kb));
+    }
+    BTRY(make_stage(b, b->yg, ya, yg));
+    BTRY(make_stage(b, b->nl, na, ng));
+    for (int s = 0; s < 3; ++s) {
+      std::vector<SolveArgs> sa;
+      for (int r = 0; r < nrun; ++r) {
+        jsd_plan* p = b->plans[r];
+        const Roles r0 = roles_of(p);
+        sa.push_back(solve_args(p, b->state[r], s, 0.0, (s & 1) ? roles_swapped(r0) : r0));
+      }
+      BTRY(make_stage(b, b->solve[s], sa, sg));
+    }
+    if (b->vjp) {
+      std::vector<NlLinTArgs> nta;
+      std::vector<YgemmArgs> aa;
+      for (int r = 0; r < nrun; ++r) {
+        jsd_plan* p = b->plans[r];
+        nta.push_back(nl_lin_T_args(p, b->adj[r]));
+        aa.push_back(acc_args(p, b->adj[r]));
+      }
+      BTRY(make_stage(b, b->nlT, nta, ng));
+      BTRY(make_stage(b, b->acc, aa, yg));
+      for (int c = 0; c < 3; ++c) {
+        const int s = 2 - c;
+        std::vector<SolveTArgs> ta;
+        std::vector<YgemmArgs> ha;
+        for (int r = 0; r < nrun; ++r) {
+          jsd_plan* p = b->plans[r];
+          const Roles r0 = roles_of(p);
+          const Roles rr = (c & 1) ? roles_swapped(r0) : r0;
+          ta.push_back(solve_T_args(p, b->adj[r], s, c > 0, rr));
+          ha.push_back(hvt_args(p, rr));
+        }
+        BTRY(make_stage(b, b->solveT[c], ta, sg));
+        BTRY(make_stage(b, b->hvt[c], ha, yg));
+      }
+    }
+  }
+  b->total[BS_YG] = b->yg.total;
+  b->total[BS_NL] = b->nl.total;
+  b->total[BS_SOLVE] = b->solve[0].total;
+  b->total[BS_SOLVET] = b->vjp ? b->solveT[0].total : 0;
+  b->total[BS_HVT] = b->vjp ? b->hvt[0].total : 0;
+  b->total[BS_NLT] = b->vjp ? b->nlT.total : 0;
+  b->total[BS_ACC] = b->vjp ? b->acc.total : 0;
+  for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = std::max(1, b->total[k]);
+  // autotune the persistent grid of each stage on the real data, then put
+  // the states (and adjoints) back as they were
+  {
+    std::vector<size_t> bytes(nrun);
+    size_t tot = 0;
+    for (int r = 0; r < nrun; ++r) {
+      bytes[r] = 3 * b->plans[r]->fs * sizeof(double2);
+      tot += bytes[r];
+    }
+    void* save = nullptr;
+    const size_t nsave = b->vjp ? 2 * tot : tot;
+    if (hipMalloc(&save, nsave) != hipSuccess) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_NOMEM, "hipMalloc(%zu) failed for the autotuner", nsave);
+    }
+    size_t off = 0;
+    int rc = JSD_OK;
+    for (int r = 0; r < nrun && !rc; ++r) {
+      if (hipMemcpyAsync((char*)save + off, b->state[r], bytes[r], hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rc = JSD_ERR_HIP;
+      if (b->vjp && hipMemcpyAsync((char*)save + tot + off, b->adj[r], bytes[r], hipMemcpyDeviceToDevice, st) !=
+                        hipSuccess)
+        rc = JSD_ERR_HIP;
+      off += bytes[r];
+    }
+    if (!rc) {
+      tune_stage<OpYgemmPlain>(b, BS_YG, &b->yg, 1, st);
+      tune_stage<OpNlLin>(b, BS_NL, &b->nl, 1, st);
+      tune_stage<OpSolve>(b, BS_SOLVE, b->solve, 3, st);
+      if (b->vjp) {
+        tune_stage<OpSolveT>(b, BS_SOLVET, b->solveT, 3, st);
+        tune_stage<OpYgemmHVT>(b, BS_HVT, b->hvt, 3, st);
+        tune_stage<OpNlLinT>(b, BS_NLT, &b->nlT, 1, st);
+        tune_stage<OpYgemmAcc>(b, BS_ACC, &b->acc, 1, st);
+      }
+    }
+    off = 0;
+    for (int r = 0; r < nrun && !rc; ++r) {
+      if (hipMemcpyAsync(b->state[r], (char*)save + off, bytes[r], hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rc = JSD_ERR_HIP;
+      if (b->vjp && hipMemcpyAsync(b->adj[r], (char*)save + tot + off, bytes[r], hipMemcpyDeviceToDevice, st) !=
+                        hipSuccess)
+        rc = JSD_ERR_HIP;
+      off += bytes[r];
+    }
+    const hipError_t es = hipStreamSynchronize(st);
+    (void)hipFree(save);
+    if (rc || es != hipSuccess) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_HIP, "autotuner copies failed");
+    }
+  }
+  // one step forward (and adjoint) captured once, replayed per step
+  {
+    hipError_t e = hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    BTRY(capture(b, true, &b->gf, &b->xf));
+    if (b->vjp) BTRY(capture(b, false, &b->gb, &b->xb));
+  }
+#undef BTRY
+  *out = b;
+  return JSD_OK;
+}
+
+extern "C" int jsd_batch_step(jsd_batch* b, int nsteps, void* stream) {
+  if (!b || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  for (int n = 0; n < nsteps; ++n) HIPCHK(hipGraphLaunch(b->xf, (hipStream_t)stream));
+  return JSD_OK;
+}
+
+extern "C" int jsd_batch_step_vjp(jsd_batch* b, int nsteps, void* stream) {
+  if (!b || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (!b->vjp) JSD_FAIL(JSD_ERR_STATE, "batch created without adjoint buffers");
+  for (int n = 0; n < nsteps; ++n) HIPCHK(hipGraphLaunch(b->xb, (hipStream_t)stream));
+  return JSD_OK;
+}
+
+extern "C" int jsd_batch_info(jsd_batch* b, int* grid, int* blocks) {
+  if (!b) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  for (int k = 0; k < BS_NKIND; ++k) {
+    if (grid) grid[k] = std::min(b->cap[k], b->total[k]);
+    if (blocks) blocks[k] = b->total[k];
+  }
   return JSD_OK;
 }
