@@ -124,7 +124,6 @@ def batch_bench(nsteps: int) -> dict:
     t0 = time.time()
     batch = BatchPlan(plans, ub)
     t_tune = time.time() - t0
-    info = batch.info()
     dev_sync()
     t0 = time.perf_counter()
     batch.step(nsteps)
@@ -145,6 +144,7 @@ def batch_bench(nsteps: int) -> dict:
     J, G = bg(u0, nsteps)
     dev_sync()
     tg_bat = time.perf_counter() - t0
+    ginfo = bg.batch.info()
     bg.close()
     gdiff = max(float(torch.linalg.norm(a - b[1]) / torch.linalg.norm(b[1])) for a, b in zip(G, ref))
     jdiff = max(abs(a - b[0]) / abs(b[0]) for a, b in zip(J, ref))
@@ -160,8 +160,8 @@ def batch_bench(nsteps: int) -> dict:
         "max_rel_diff_state": fwd_diff,
         "max_rel_diff_grad": gdiff,
         "max_rel_diff_gain": jdiff,
-        "autotuned_grid": {k: v["grid"] for k, v in info.items()},
-        "logical_blocks": {k: v["blocks"] for k, v in info.items()},
+        "autotuned_grid": {k: v["grid"] for k, v in ginfo.items()},
+        "logical_blocks": {k: v["blocks"] for k, v in ginfo.items()},
         "plan_build_s": round(t_build, 2),
         "autotune_and_capture_s": round(t_tune, 2),
     }

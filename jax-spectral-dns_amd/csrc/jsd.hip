@@ -219,6 +219,107 @@ __global__ __launch_bounds__(256) void k_ygemm(YgemmArgs G) {
 }
 
 // ===========================================================================
+// y-derivative on even/odd parity blocks
+// ===========================================================================
+// D is centro-antisymmetric (D[N-1-i][N-1-j] = -D[i][j]) and D^2
+// centro-symmetric.  With e_h = (x_h + x_h')/2, o_h = (x_h - x_h')/2
+// (h' = N-1-h, h < ceil(N/2)) the product splits into two blocks of at most
+// 66 x 66: D gives even rows from the odd part and odd rows from the even
+// part, D^2 even from even and odd from odd; half the MFMA work of the full
+// Ny x Ny product.  Columns are the flattened (kx row, kz re/im) index, so no
+// workgroup is left with a ragged tail; a wave owns 16 columns and all row
+// tiles; each block's A fragments are staged in LDS once per workgroup.
+#define YP_TMAX 5   // 16-row tiles per half (N <= 132)
+#define YP_KMAX 17  // 4-row k-steps per half
+
+struct YparArgs {
+  const double* F;  // fragments: even-output block (ET tiles x KIE k-steps), then odd-output (OT x KIO)
+  int ET, OT, KIE, KIO, KE, KO;  // tiles per output half; k-steps feeding each; k-steps of the even / odd input
+  int nEf, nOf;                  // half sizes (walls included): ceil(N/2), floor(N/2)
+  int anti;                      // 1: D (parities swap), 0: D^2
+  const double* in;
+  double* out;
+  int M, N, Z;
+  long comp_stride;  // doubles between components
+};
+
+// block bx of 64 flattened columns, component bz; sF: dynamic LDS of
+// max(ET*KIE, OT*KIO)*64 doubles
+__device__ __forceinline__ void ypar_body(const YparArgs& G, int bx, int bz, double* sF) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kr = lane >> 4, cc = lane & 15;
+  const int N = G.N, C2 = 2 * G.Z;
+  const long gcol = (long)bx * 64 + wave * 16 + cc;
+  const bool ok = gcol < (long)G.M * C2;
+  const long i = ok ? gcol / C2 : 0;
+  const int c = ok ? (int)(gcol - i * C2) : 0;
+  const size_t base = (size_t)bz * G.comp_stride + (size_t)i * N * C2 + c;
+  // even-output block first, then the odd-output block, through one LDS buffer
+  const int nfE = G.ET * G.KIE * 64, nfO = G.OT * G.KIO * 64;
+  for (int it = threadIdx.x; it < nfE; it += 256) sF[it] = G.F[it];
+  // even / odd parts of this lane's column at rows h = 4 ks + kr
+  double be[YP_KMAX], bo[YP_KMAX];
+#pragma unroll
+  for (int ks = 0; ks < YP_KMAX; ++ks) {
+    const int h = 4 * ks + kr;
+    double a = 0.0, b = 0.0;
+    if (ok && h < G.nEf) {
+      a = G.in[base + (size_t)h * C2];
+      b = (N - 1 - h != h) ? G.in[base + (size_t)(N - 1 - h) * C2] : a;
+    }
+    be[ks] = (N - 1 - h == h) ? a : 0.5 * (a + b);
+    bo[ks] = (h < G.nOf) ? 0.5 * (a - b) : 0.0;
+  }
+  __syncthreads();
+  dbl4 ae[YP_TMAX], ao[YP_TMAX];
+#pragma unroll
+  for (int t = 0; t < YP_TMAX; ++t) {
+    ae[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    ao[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  }
+  const bool anti = G.anti != 0;
+#pragma unroll
+  for (int ks = 0; ks < YP_KMAX; ++ks) {
+    if (ks < G.KIE) {
+      const double b = anti ? bo[ks] : be[ks];
+#pragma unroll
+      for (int t = 0; t < YP_TMAX; ++t)
+        if (t < G.ET) ae[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sF[(t * G.KIE + ks) * 64 + lane], b, ae[t], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < nfO; it += 256) sF[it] = G.F[nfE + it];
+  __syncthreads();
+#pragma unroll
+  for (int ks = 0; ks < YP_KMAX; ++ks) {
+    if (ks < G.KIO) {
+      const double b = anti ? be[ks] : bo[ks];
+#pragma unroll
+      for (int t = 0; t < YP_TMAX; ++t)
+        if (t < G.OT) ao[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sF[(t * G.KIO + ks) * 64 + lane], b, ao[t], 0, 0, 0);
+    }
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int t = 0; t < YP_TMAX; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int h = t * 16 + 4 * r + kr;  // f64 16x16x4: reg r of lane L is row 4r + L/16
+      if (t >= G.ET || h >= G.nEf) continue;
+      const double e = ae[t][r];
+      const double o = (t < G.OT && h < G.nOf) ? ao[t][r] : 0.0;
+      G.out[base + (size_t)h * C2] = e + o;
+      if (N - 1 - h != h) G.out[base + (size_t)(N - 1 - h) * C2] = e - o;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ypar(YparArgs G) {
+  extern __shared__ double sF[];
+  ypar_body(G, blockIdx.x, blockIdx.z, sF);
+}
+
+// ===========================================================================
 // FFT passes (nonlinear path)
 // ===========================================================================
 // A WG of the x-passes owns QB kz "pairs" q = q0..q0+QB-1 (q <= KZ); slot c
@@ -534,6 +635,7 @@ __global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, int ntr, int n
   const double2* __restrict__ A = set.A[tr];
   const double2* __restrict__ B = set.B[tr];
   const int opA = set.opA[tr], opB = set.opB[tr];
+  const bool sameAB = (A == B);  // (u_c, d_x u_c): one field read once
   const int K = M / 2, KZ = Z / 2;
   for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
     const int c = w % NSLOT;
@@ -550,10 +652,12 @@ __global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, int ntr, int n
       const int pj = (Z - j) % Z;
       const size_t e = ((size_t)i * N + l) * Z + j;
       const size_t ep = ((size_t)pi * N + l) * Z + pj;
-      const double2 a = apply_op(opA, A[e], kx[i], kz[j]);
-      const double2 ap = apply_op(opA, A[ep], kx[pi], kz[pj]);
-      const double2 b = apply_op(opB, B[e], kx[i], kz[j]);
-      const double2 bp = apply_op(opB, B[ep], kx[pi], kz[pj]);
+      const double2 ra = A[e], rap = A[ep];
+      const double2 rb = sameAB ? ra : B[e], rbp = sameAB ? rap : B[ep];
+      const double2 a = apply_op(opA, ra, kx[i], kz[j]);
+      const double2 ap = apply_op(opA, rap, kx[pi], kz[pj]);
+      const double2 b = apply_op(opB, rb, kx[i], kz[j]);
+      const double2 bp = apply_op(opB, rbp, kx[pi], kz[pj]);
       const double2 ha = cherm(a, ap);
       const double2 hb = cherm(b, bp);
       val = make_double2((ha.x - hb.y) * s_inv, (ha.y + hb.x) * s_inv);
@@ -627,6 +731,124 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
     const int n = (j <= KZ) ? j : P - (Z - j);
     V[line + j] = lds[n * 2];
     V[vstride + line + j] = lds[n * 2 + 1];
+  }
+}
+
+// z pass for P = 256 = 16 x 16 with the butterflies in registers.  With
+// n = n1 + 16 n2 and k = 16 k1 + k2: step 1 (one thread per row t and n1)
+// reads x[n1 + 16 n2] straight from HBM (zero padding by index), takes the
+// 16-point DFT over n2 and multiplies by w256^(n1 k2); step 2 (one thread per
+// t and k2) takes the 16-point DFT over n1, giving the physical point
+// z = 16 k1 + k2.  Two LDS exchanges per transform instead of three stages,
+// no LDS zero fill.  The forward transform of the two packed product rows
+// runs the same two steps and stores only the kept bins.
+#define Z16_LD 17  // padded row (in double2) of the [t][n1][k2] exchange layout
+
+template <int NT>
+__host__ __device__ constexpr int z256_threads() { return NT * 16 > 128 ? NT * 16 : 128; }
+
+template <int NT, int PROD>
+__global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
+    const double2* __restrict__ T, long tstride, double2* __restrict__ V, long vstride,
+    const double2* __restrict__ tw, int N, int Z, int ZT, double clin, const double* __restrict__ Ub,
+    const double* __restrict__ dUb) {
+  constexpr int P = 256;
+  constexpr int NTH = z256_threads<NT>();
+  constexpr int ITEMS = (P + NTH - 1) / NTH;
+  // [t][n1][k2] (padded) for the exchanges, [z][t] for the products
+  __shared__ double2 lds[NT * 16 * Z16_LD > P * NT ? NT * 16 * Z16_LD : P * NT];
+  const int x = blockIdx.x;
+  const int l = blockIdx.y;
+  const int KZ = Z / 2;
+  const int nneg = Z - KZ - 1;  // stored negative bins
+  const size_t line = ((size_t)x * N + l) * ZT;
+  double2 v[16];
+  // ---- inverse, step 1
+  if (threadIdx.x < NT * 16) {
+    const int t = threadIdx.x >> 4, n1 = threadIdx.x & 15;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const int n = n1 + 16 * n2;
+      int j = -1;
+      if (n <= KZ) j = n;
+      else if (n >= P - nneg) j = n - P + Z;
+      v[n2] = j >= 0 ? T[t * tstride + line + j] : make_double2(0.0, 0.0);
+    }
+    dft16<+1>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      double2 y = v[k2];
+      if (n1 * k2 > 0) y = cmul(y, cconj(tw[n1 * k2]));
+      lds[(t * 16 + n1) * Z16_LD + k2] = y;
+    }
+  }
+  __syncthreads();
+  // ---- inverse, step 2
+  if (threadIdx.x < NT * 16) {
+    const int t = threadIdx.x >> 4, k2 = threadIdx.x & 15;
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[(t * 16 + n1) * Z16_LD + k2];
+    dft16<+1>(v);
+  }
+  __syncthreads();
+  if (threadIdx.x < NT * 16) {
+    const int t = threadIdx.x >> 4, k2 = threadIdx.x & 15;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) lds[(16 * k1 + k2) * NT + t] = v[k1];
+  }
+  __syncthreads();
+  // ---- products at each physical point
+  const double U = Ub ? Ub[l] : 0.0;
+  const double dU = dUb ? dUb[l] : 0.0;
+  double Hs[ITEMS][3];
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const int z = threadIdx.x + q * NTH;
+    if (z < P) zmid_products<PROD>(lds + z * NT, U, dU, clin, Hs[q]);
+  }
+  __syncthreads();
+  // packed rows (H0 + i H1) and (H2 + 0 i) at [o][z]
+#pragma unroll
+  for (int q = 0; q < ITEMS; ++q) {
+    const int z = threadIdx.x + q * NTH;
+    if (z < P) {
+      lds[z] = make_double2(Hs[q][0], Hs[q][1]);
+      lds[P + z] = make_double2(Hs[q][2], 0.0);
+    }
+  }
+  __syncthreads();
+  // ---- forward, step 1 (rows o = 0, 1)
+  if (threadIdx.x < 32) {
+    const int o = threadIdx.x >> 4, n1 = threadIdx.x & 15;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = lds[o * P + n1 + 16 * n2];
+    dft16<-1>(v);
+  }
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const int o = threadIdx.x >> 4, n1 = threadIdx.x & 15;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      double2 y = v[k2];
+      if (n1 * k2 > 0) y = cmul(y, tw[n1 * k2]);
+      lds[(o * 16 + n1) * Z16_LD + k2] = y;
+    }
+  }
+  __syncthreads();
+  // ---- forward, step 2: kept bins to V
+  if (threadIdx.x < 32) {
+    const int o = threadIdx.x >> 4, k2 = threadIdx.x & 15;
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[(o * 16 + n1) * Z16_LD + k2];
+    dft16<-1>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+      const int b = 16 * k1 + k2;
+      int j = -1;
+      if (b <= KZ) j = b;
+      else if (b >= P - nneg) j = b - P + Z;
+      if (j >= 0) V[o * vstride + line + j] = v[k1];
+    }
   }
 }
 
@@ -956,6 +1178,8 @@ struct jsd_plan {
   std::vector<void*> allocs;
   // device tables
   double *DF1, *DF2, *DFT1, *D_R, *D2_R, *RB_R, *LI_R, *PO_R;  // adjoint: D^T tiles, row-major tables
+  double *YF1, *YF2;  // parity-block fragments of D and D^2 for k_ypar
+  YparArgs yp1, yp2;  // their block geometry
   double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
   double *DEO, *DOE, *D2E, *D2O, *RBP, *LIP, *POP, *CW, *RW, *VAP, *VINV;  // parity blocks of the solve
   double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
@@ -1119,6 +1343,39 @@ static void parity_frag(const double* A, int N, int sgn, int qi, int qo, double*
   parity_tile(blk.data(), frag);
 }
 
+// Fragments of the full-row parity blocks of A (walls included: half index h
+// is row h and its mirror N-1-h), sgn = -1 for D (parities swap), +1 for D^2.
+static void ypar_frags(const double* A, int N, int sgn, std::vector<double>& F, YparArgs& G) {
+  const int nEf = (N + 1) / 2, nOf = N / 2;
+  G.nEf = nEf;
+  G.nOf = nOf;
+  G.KE = (nEf + 3) / 4;
+  G.KO = (nOf + 3) / 4;
+  G.ET = (nEf + 15) / 16;
+  G.OT = (nOf + 15) / 16;
+  G.anti = sgn < 0 ? 1 : 0;
+  G.KIE = G.anti ? G.KO : G.KE;
+  G.KIO = G.anti ? G.KE : G.KO;
+  F.assign((size_t)(G.ET * G.KIE + G.OT * G.KIO) * 64, 0.0);
+  auto entry = [&](int h, int qi, int g) {
+    const int m = g, mp = N - 1 - g;
+    if (m == mp) return sym_entry(A, N, sgn, h, m);
+    return sym_entry(A, N, sgn, h, m) + (qi ? -1.0 : 1.0) * sym_entry(A, N, sgn, h, mp);
+  };
+  for (int qo = 0; qo < 2; ++qo) {
+    const int qi = G.anti ? 1 - qo : qo;
+    const int no = qo ? nOf : nEf, ni = qi ? nOf : nEf;
+    const int T = qo ? G.OT : G.ET, K = qo ? G.KIO : G.KIE;
+    double* f = F.data() + (qo ? (size_t)G.ET * G.KIE * 64 : 0);
+    for (int t = 0; t < T; ++t)
+      for (int ks = 0; ks < K; ++ks)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int h = t * 16 + (lane & 15), g = ks * 4 + (lane >> 4);
+          f[((size_t)t * K + ks) * 64 + lane] = (h < no && g < ni) ? entry(h, qi, g) : 0.0;
+        }
+  }
+}
+
 extern "C" int jsd_version(void) { return 1; }
 extern "C" const char* jsd_last_error(void) { return g_err.c_str(); }
 
@@ -1188,6 +1445,10 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   TRY(upload(p, &p->DF1, tiled.data(), tiled.size()));
   tile_D(D2.data(), N, p->NT, p->KS, tiled);
   TRY(upload(p, &p->DF2, tiled.data(), tiled.size()));
+  ypar_frags(d->D, N, -1, tiled, p->yp1);
+  TRY(upload(p, &p->YF1, tiled.data(), tiled.size()));
+  ypar_frags(D2.data(), N, +1, tiled, p->yp2);
+  TRY(upload(p, &p->YF2, tiled.data(), tiled.size()));
   {
     // parity blocks: D (centro-antisymmetric), D^2, rhs_base[s], lhs_inv[s][k],
     // poisson[k] (centro-symmetric), wall rows/columns, v_a in parity form
@@ -1368,11 +1629,29 @@ static YgemmArgs ygemm_args(const jsd_plan* p, const double* DF, const void* in0
 }
 static dim3 ygemm_grid(const jsd_plan* p, int ncomp) { return dim3(p->M, (2 * p->Z + 63) / 64, ncomp); }
 
+// D^order along y on the parity blocks: arguments, number of 64-column
+// blocks and the dynamic LDS it needs (kept as the maximum in *lds)
+static YparArgs ypar_args(const jsd_plan* p, const void* in, void* out, int order, unsigned* nblk, size_t* lds) {
+  YparArgs G = order == 2 ? p->yp2 : p->yp1;
+  G.F = order == 2 ? p->YF2 : p->YF1;
+  G.in = (const double*)in;
+  G.out = (double*)out;
+  G.M = p->M;
+  G.N = p->N;
+  G.Z = p->Z;
+  G.comp_stride = (long)(2 * p->fs);
+  const long ncol = (long)p->M * 2 * p->Z;
+  *nblk = (unsigned)((ncol + 63) / 64);
+  *lds = std::max(*lds, (size_t)std::max(G.ET * G.KIE, G.OT * G.KIO) * 64 * sizeof(double));
+  return G;
+}
+
 static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int ncomp, int order, hipStream_t st) {
+  unsigned nblk = 0;
+  size_t lds = 0;
+  const YparArgs G = ypar_args(p, in, out, order, &nblk, &lds);
   prof_begin(p, 0, st);
-  hipLaunchKernelGGL(k_ygemm<GEMM_PLAIN>, ygemm_grid(p, ncomp), dim3(256), 0, st,
-                     ygemm_args(p, order == 2 ? p->DF2 : p->DF1, in, nullptr, nullptr, out, nullptr, nullptr, nullptr,
-                                (long)(2 * p->fs)));
+  hipLaunchKernelGGL(k_ypar, dim3(nblk, 1, ncomp), dim3(256), lds, st, G);
   prof_end(p, 0, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
@@ -1474,7 +1753,20 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   prof_end(p, 1, st);
   HIPCHK(hipGetLastError());
   prof_begin(p, 2, st);
-  const bool z_ct = ct && visit_z(p->fpz, [&](auto c) {
+  const bool z_256 = ct && plan_is<RZ256>(p->fpz);
+  if (z_256) {
+    const dim3 grid(d.ppx, N);
+    if (prod == PROD_NS)
+      hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_NS>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts, p->V,
+                         (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+    else if (prod == PROD_PERT)
+      hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_PERT>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts, p->V,
+                         (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+    else
+      hipLaunchKernelGGL((k_fft_z_mid_256<12, PROD_DUAL>), grid, dim3(z256_threads<12>()), 0, st, p->T, (long)ts,
+                         p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+  }
+  const bool z_ct = z_256 || (ct && visit_z(p->fpz, [&](auto c) {
     using C = decltype(c);
     const size_t lds = (size_t)C::P * ntr * sizeof(double2);
     const dim3 grid(d.ppx, N), block(zmid_threads(C::P));
@@ -1487,7 +1779,7 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     else
       hipLaunchKernelGGL((k_fft_z_mid_t<C, 12, PROD_DUAL>), grid, block, lds, st, p->T, (long)ts, p->V, (long)ts,
                          p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
-  });
+  }));
   if (!z_ct) {
     // y planes per z-pass workgroup: LDS <= 80 KB and <= ZMID_ITEMS items per thread
     int YB = (int)((80 * 1024) / ((size_t)d.ppz * ntr * sizeof(double2)));
@@ -1784,10 +2076,6 @@ extern "C" int jsd_energy_grad(jsd_plan* p, const void* vel_hat, void* out, doub
 // per-run grids (run r owns logical blocks [first_r, first_{r+1})) with a
 // persistent grid of any size.
 
-struct OpYgemmPlain {
-  typedef YgemmArgs Args;
-  static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_PLAIN>(a, k); }
-};
 struct OpYgemmHVT {
   typedef YgemmArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_HVT>(a, k); }
@@ -1795,6 +2083,13 @@ struct OpYgemmHVT {
 struct OpYgemmAcc {
   typedef YgemmArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_ACC>(a, k); }
+};
+struct OpYpar {
+  typedef YparArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) {
+    extern __shared__ double sF[];
+    ypar_body(a, k.x, k.z, sF);
+  }
 };
 struct OpNlLin {
   typedef NlLinArgs Args;
@@ -1837,6 +2132,7 @@ struct BatchStage {
   void* args = nullptr;
   int4* runs = nullptr;
   int total = 0;
+  size_t lds = 0;  // dynamic LDS bytes of the stage kernel
 };
 
 struct jsd_batch {
@@ -1882,7 +2178,7 @@ static int make_stage(jsd_batch* b, BatchStage& S, const std::vector<A>& args, c
 template <class Op>
 static int launch_stage(const jsd_batch* b, const BatchStage& S, int cap, hipStream_t st) {
   const int grid = std::max(1, std::min(S.total, cap));
-  hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(256), 0, st, (const typename Op::Args*)S.args, S.runs,
+  hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(256), S.lds, st, (const typename Op::Args*)S.args, S.runs,
                      b->nrun, S.total);
   HIPCHK(hipGetLastError());
   return JSD_OK;
@@ -1890,7 +2186,7 @@ static int launch_stage(const jsd_batch* b, const BatchStage& S, int cap, hipStr
 
 static int batch_fwd_step(const jsd_batch* b, hipStream_t st) {
   for (int s = 0; s < 3; ++s) {
-    int rc = launch_stage<OpYgemmPlain>(b, b->yg, b->cap[BS_YG], st);
+    int rc = launch_stage<OpYpar>(b, b->yg, b->cap[BS_YG], st);
     if (!rc) rc = launch_stage<OpNlLin>(b, b->nl, b->cap[BS_NL], st);
     if (!rc) rc = launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
     if (rc) return rc;
@@ -2014,20 +2310,23 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   // forward stages; the carry buffers alternate with the substep (any
   // assignment is valid at step boundaries: substep 0 reads no old terms)
   {
-    std::vector<YgemmArgs> ya;
+    std::vector<YparArgs> ya;
     std::vector<NlLinArgs> na;
-    std::vector<dim3> yg, ng, sg;
+    std::vector<dim3> yg, yp, ng, sg;
+    size_t ylds = 0;
     for (int r = 0; r < nrun; ++r) {
       jsd_plan* p = b->plans[r];
       double2* U = b->state[r];
-      ya.push_back(ygemm_args(p, p->DF1, U + p->fs, nullptr, nullptr, p->DU, nullptr, nullptr, nullptr,
-                              (long)(2 * p->fs)));
+      unsigned nyb = 0;
+      ya.push_back(ypar_args(p, U + p->fs, p->DU, 1, &nyb, &ylds));
+      yp.push_back(dim3(nyb));
       yg.push_back(ygemm_grid(p, 1));
       na.push_back(nl_lin_args(p, U));
       ng.push_back(dim3((unsigned)((p->fs + 255) / 256)));
       sg.push_back(dim3(p->Ka * p->Kb));
     }
-    BTRY(make_stage(b, b->yg, ya, yg));
+    BTRY(make_stage(b, b->yg, ya, yp));
+    b->yg.lds = ylds;
     BTRY(make_stage(b, b->nl, na, ng));
     for (int s = 0; s < 3; ++s) {
       std::vector<SolveArgs> sa;
@@ -2098,7 +2397,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       off += bytes[r];
     }
     if (!rc) {
-      tune_stage<OpYgemmPlain>(b, BS_YG, &b->yg, 1, st);
+      tune_stage<OpYpar>(b, BS_YG, &b->yg, 1, st);
       tune_stage<OpNlLin>(b, BS_NL, &b->nl, 1, st);
       tune_stage<OpSolve>(b, BS_SOLVE, b->solve, 3, st);
       if (b->vjp) {

@@ -92,6 +92,44 @@ __device__ __forceinline__ void fft_stages_ct(double2* buf, const double2* __res
   }
 }
 
+// 16-point DFT in registers, natural order in and out (n = n1 + 4 n2,
+// k = 4 k1 + k2: four radix-4 DFTs, twiddles w16^(n1 k2), four radix-4 DFTs).
+// Twiddles: cos / sin of 2 pi m / 16, correctly rounded.
+struct W16 {
+  static constexpr double c[16] = {1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898,
+                                   0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867,
+                                   -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898,
+                                   0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867};
+  static constexpr double s[16] = {0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867,
+                                   1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898,
+                                   0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867,
+                                   -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898};
+};
+
+template <int SIGN>
+__device__ __forceinline__ void dft16(double2* v) {
+  constexpr double sign = (double)SIGN;
+  double2 a[4][4];  // a[n1][k2]
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    double2 t[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    dft4(t, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      const int m = n1 * k2;
+      if (m == 0) a[n1][k2] = t[k2];
+      else a[n1][k2] = cmul(t[k2], make_double2(W16::c[m], sign * W16::s[m]));
+    }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    double2 t[4] = {a[0][k2], a[1][k2], a[2][k2], a[3][k2]};
+    dft4(t, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[4 * k1 + k2] = t[k1];
+  }
+}
+
 // Full in-place FFT of NROWS rows already in DIT order; SIGN -1 forward, +1 inverse.
 template <class C, int NROWS, int PS, int RS, int NTH, int SIGN>
 __device__ __forceinline__ void fft_lds_ct(double2* buf, const double2* __restrict__ tw) {

@@ -93,4 +93,4 @@ def gain_and_grad(u0: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_ever
     x = torch.view_as_real(u0.contiguous()).clone().requires_grad_(True)
     J = energy(run(x, plan, nsteps, checkpoint_every), plan) / energy(x, plan)
     (gx,) = torch.autograd.grad(J, x)
-    return float(J), torch.view_as_complex(gx.contiguous())
+    return float(J.detach()), torch.view_as_complex(gx.contiguous())
