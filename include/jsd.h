@@ -41,7 +41,7 @@ typedef struct jsd_plan jsd_plan;
  * operator tables come from the host precompute (one entry per distinct
  * (|kx|, |kz|) pair, Ka = nxm/2+1, Kb = nzm/2+1, row-major). */
 typedef struct jsd_plan_desc {
-  int nxm, ny, nzm;   /* stored modes (odd, = 1 mod 4) and Chebyshev points (<= 144) */
+  int nxm, ny, nzm;   /* stored modes (odd, = 1 mod 4) and Chebyshev points (<= 130) */
   int px, pz;         /* the reference's physical grid: fixes the spectral scaling */
   int ppx, ppz;       /* transform grid of the FFT kernels: >= 3K+1, radices 2,3,4,5,7,8 */
   double lx, lz;      /* box lengths */
@@ -62,6 +62,8 @@ typedef struct jsd_plan_desc {
   const double* base_u;     /* ny base-flow profile U(y) (perturbation / dual) */
   const double* base_du;    /* ny U'(y) */
   const double* y_weights;  /* ny definite-integral weights */
+  int adjoint;              /* nonzero: also build the tables of the discrete adjoint
+                               (always built for linearised perturbation plans) */
 } jsd_plan_desc;
 
 /* Replaces NavierStokesVelVort.__init__ / assemble_matrices
@@ -101,6 +103,19 @@ int jsd_energy(jsd_plan* plan, const void* vel_hat, double* out, void* stream);
  * jax.value_and_grad in optimiser.py:115-121 (navier_stokes.py:1839-1851).
  * Only for JSD_KIND_PERTURBATION plans with linearise = 1. */
 int jsd_step_vjp(jsd_plan* plan, void* grad, int nsteps, void* stream);
+
+/* One step of the discrete adjoint of a nonlinear step (JSD_KIND_NS, or
+ * JSD_KIND_PERTURBATION with linearise = 0; the plan must be created with
+ * adjoint = 1): grad holds the adjoint of the state after the step on entry
+ * and the adjoint of the state before it on exit; state is the forward state
+ * at the start of the step (unchanged) and dPdx the step's pressure
+ * gradient.  The three substep states are recomputed from state and the
+ * nonlinear term is transposed about each of them (transposed transforms and
+ * products).  Replaces reverse-mode AD through the nonlinear scan
+ * (jax.value_and_grad in optimiser.py:115-121).  The first call allocates
+ * 9 spectral fields of recompute storage, kept by the plan.  Linearised
+ * perturbation plans ignore state and run jsd_step_vjp. */
+int jsd_step_vjp_state(jsd_plan* plan, const void* state, void* grad, double dPdx, void* stream);
 
 /* out = scale * dE/d(vel_hat) on the real-pair view, E the energy of
  * jsd_energy (field.py:607-611). */

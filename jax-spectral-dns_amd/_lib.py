@@ -47,6 +47,7 @@ class PlanDesc(ctypes.Structure):
         ("base_u", _dp),
         ("base_du", _dp),
         ("y_weights", _dp),
+        ("adjoint", ctypes.c_int),
     ]
 
 
@@ -66,6 +67,7 @@ EXPORTS = (
     "jsd_energy",
     "jsd_ydiff",
     "jsd_step_vjp",
+    "jsd_step_vjp_state",
     "jsd_energy_grad",
     "jsd_profile",
     "jsd_batch_create",
@@ -94,6 +96,7 @@ def load() -> ctypes.CDLL:
     lib.jsd_energy.argtypes = [vp, vp, _dp, vp]
     lib.jsd_ydiff.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     lib.jsd_step_vjp.argtypes = [vp, vp, ctypes.c_int, vp]
+    lib.jsd_step_vjp_state.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
     lib.jsd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, vp,

@@ -243,71 +243,83 @@ struct YparArgs {
   long comp_stride;  // doubles between components
 };
 
-// block bx of 64 flattened columns, component bz; sF: dynamic LDS of
-// max(ET*KIE, OT*KIO)*64 doubles
-__device__ __forceinline__ void ypar_body(const YparArgs& G, int bx, int bz, double* sF) {
+// Block bx of 32 flattened columns, component bz.  Waves 0, 1 compute the
+// even output rows of columns 0..15 / 16..31, waves 2, 3 the odd rows of
+// the same columns; each wave reads its block's A fragments straight from
+// L2 (prefetched one k-step ahead) and its input parity from HBM.  The odd
+// halves pass through LDS (sO: 80 x 32 doubles) to the even waves, which
+// write out[h] = e + o and out[N-1-h] = e - o.
+__device__ __forceinline__ void ypar_body(const YparArgs& G, int bx, int bz, double* sO) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kr = lane >> 4, cc = lane & 15;
+  const int half = wave >> 1;               // 0: even output rows, 1: odd
+  const int lc = (wave & 1) * 16 + cc;      // column within the block
   const int N = G.N, C2 = 2 * G.Z;
-  const long gcol = (long)bx * 64 + wave * 16 + cc;
+  const long gcol = (long)bx * 32 + lc;
   const bool ok = gcol < (long)G.M * C2;
   const long i = ok ? gcol / C2 : 0;
   const int c = ok ? (int)(gcol - i * C2) : 0;
   const size_t base = (size_t)bz * G.comp_stride + (size_t)i * N * C2 + c;
-  // even-output block first, then the odd-output block, through one LDS buffer
-  const int nfE = G.ET * G.KIE * 64, nfO = G.OT * G.KIO * 64;
-  for (int it = threadIdx.x; it < nfE; it += 256) sF[it] = G.F[it];
-  // even / odd parts of this lane's column at rows h = 4 ks + kr
-  double be[YP_KMAX], bo[YP_KMAX];
+  const bool anti = G.anti != 0;
+  // input parity feeding this half: D swaps parities, D^2 keeps them
+  const int qin = anti ? 1 - half : half;
+  const int T = half ? G.OT : G.ET;
+  const int K = half ? G.KIO : G.KIE;
+  const double* __restrict__ F = G.F + (half ? (size_t)G.ET * G.KIE * 64 : 0) + lane;
+  double bx_[YP_KMAX];
 #pragma unroll
   for (int ks = 0; ks < YP_KMAX; ++ks) {
     const int h = 4 * ks + kr;
-    double a = 0.0, b = 0.0;
-    if (ok && h < G.nEf) {
-      a = G.in[base + (size_t)h * C2];
-      b = (N - 1 - h != h) ? G.in[base + (size_t)(N - 1 - h) * C2] : a;
+    double v = 0.0;
+    if (ok && ks < K && h < (qin ? G.nOf : G.nEf)) {
+      const double xa = G.in[base + (size_t)h * C2];
+      const int hm = N - 1 - h;
+      if (hm == h) v = qin ? 0.0 : xa;
+      else {
+        const double xb = G.in[base + (size_t)hm * C2];
+        v = qin ? 0.5 * (xa - xb) : 0.5 * (xa + xb);
+      }
     }
-    be[ks] = (N - 1 - h == h) ? a : 0.5 * (a + b);
-    bo[ks] = (h < G.nOf) ? 0.5 * (a - b) : 0.0;
+    bx_[ks] = v;
   }
-  __syncthreads();
-  dbl4 ae[YP_TMAX], ao[YP_TMAX];
+  dbl4 acc[YP_TMAX];
 #pragma unroll
-  for (int t = 0; t < YP_TMAX; ++t) {
-    ae[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
-    ao[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  }
-  const bool anti = G.anti != 0;
+  for (int t = 0; t < YP_TMAX; ++t) acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  double an[YP_TMAX];
+#pragma unroll
+  for (int t = 0; t < YP_TMAX; ++t) an[t] = (t < T && K > 0) ? F[(size_t)(t * K) * 64] : 0.0;
 #pragma unroll
   for (int ks = 0; ks < YP_KMAX; ++ks) {
-    if (ks < G.KIE) {
-      const double b = anti ? bo[ks] : be[ks];
+    if (ks < K) {
+      double a[YP_TMAX];
+#pragma unroll
+      for (int t = 0; t < YP_TMAX; ++t) a[t] = an[t];
+#pragma unroll
+      for (int t = 0; t < YP_TMAX; ++t) an[t] = (t < T && ks + 1 < K) ? F[(size_t)(t * K + ks + 1) * 64] : 0.0;
 #pragma unroll
       for (int t = 0; t < YP_TMAX; ++t)
-        if (t < G.ET) ae[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sF[(t * G.KIE + ks) * 64 + lane], b, ae[t], 0, 0, 0);
+        if (t < T) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], bx_[ks], acc[t], 0, 0, 0);
     }
   }
-  __syncthreads();
-  for (int it = threadIdx.x; it < nfO; it += 256) sF[it] = G.F[nfE + it];
-  __syncthreads();
+  if (half == 1) {
 #pragma unroll
-  for (int ks = 0; ks < YP_KMAX; ++ks) {
-    if (ks < G.KIO) {
-      const double b = anti ? be[ks] : bo[ks];
+    for (int t = 0; t < YP_TMAX; ++t)
 #pragma unroll
-      for (int t = 0; t < YP_TMAX; ++t)
-        if (t < G.OT) ao[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sF[(t * G.KIO + ks) * 64 + lane], b, ao[t], 0, 0, 0);
-    }
+      for (int r = 0; r < 4; ++r) {
+        const int h = t * 16 + 4 * r + kr;
+        if (t < T) sO[h * 32 + lc] = acc[t][r];
+      }
   }
-  if (!ok) return;
+  __syncthreads();
+  if (half == 1 || !ok) return;
 #pragma unroll
   for (int t = 0; t < YP_TMAX; ++t) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int h = t * 16 + 4 * r + kr;  // f64 16x16x4: reg r of lane L is row 4r + L/16
-      if (t >= G.ET || h >= G.nEf) continue;
-      const double e = ae[t][r];
-      const double o = (t < G.OT && h < G.nOf) ? ao[t][r] : 0.0;
+      if (t >= T || h >= G.nEf) continue;
+      const double e = acc[t][r];
+      const double o = (h < G.nOf) ? sO[h * 32 + lc] : 0.0;
       G.out[base + (size_t)h * C2] = e + o;
       if (N - 1 - h != h) G.out[base + (size_t)(N - 1 - h) * C2] = e - o;
     }
@@ -315,8 +327,8 @@ __device__ __forceinline__ void ypar_body(const YparArgs& G, int bx, int bz, dou
 }
 
 __global__ __launch_bounds__(256) void k_ypar(YparArgs G) {
-  extern __shared__ double sF[];
-  ypar_body(G, blockIdx.x, blockIdx.z, sF);
+  __shared__ double sO[YP_TMAX * 16 * 32];
+  ypar_body(G, blockIdx.x, blockIdx.z, sO);
 }
 
 // ===========================================================================
@@ -325,7 +337,7 @@ __global__ __launch_bounds__(256) void k_ypar(YparArgs G) {
 // A WG of the x-passes owns QB kz "pairs" q = q0..q0+QB-1 (q <= KZ); slot c
 // (0..2QB-1) holds column j = q (c < QB) or j = Z - q (c >= QB, q >= 1).
 // The partner column -kz of slot c is slot c ^ QB (slot c itself for q = 0).
-#define QB 8
+#define QB 4
 #define NSLOT (2 * QB)
 
 __device__ __forceinline__ int slot_col(int c, int q0, int Z, int KZ, bool* ok) {
@@ -562,12 +574,14 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
 // H0, H1; t=1 writes H2.
 __global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V, long vstride,
                                                    double2* __restrict__ H, long hstride, FftPlan PX, int M, int N,
-                                                   int Z, int ZT, double s_fwd) {
+                                                   int Z, int ZT, int nqb, double s_fwd) {
   extern __shared__ double2 lds[];
   double2* buf = lds;
-  const int q0 = blockIdx.x * QB;
-  const int l = blockIdx.y;
-  const int t = blockIdx.z;
+  const int g = xcd_logical();
+  if (g >= 2 * nqb * N) return;
+  const int q0 = (g % nqb) * QB;
+  const int t = (g / nqb) % 2;
+  const int l = g / (2 * nqb);
   const int K = M / 2, KZ = Z / 2;
   const int P = PX.p;
   const double2* Vt = V + t * vstride;
@@ -856,12 +870,16 @@ template <class C>
 __global__ __launch_bounds__(256) void k_fft_x_fwd_t(const double2* __restrict__ V, long vstride,
                                                      double2* __restrict__ H, long hstride,
                                                      const double2* __restrict__ tw, int M, int N, int Z, int ZT,
-                                                     double s_fwd) {
+                                                     int nqb, double s_fwd) {
   extern __shared__ double2 lds[];
   constexpr int P = C::P;
-  const int q0 = blockIdx.x * QB;
-  const int l = blockIdx.y;
-  const int t = blockIdx.z;
+  // logical order (l, t, qb), qb fastest: neighbouring kz blocks share
+  // cache lines and run on one XCD
+  const int g = xcd_logical();
+  if (g >= 2 * nqb * N) return;
+  const int q0 = (g % nqb) * QB;
+  const int t = (g / nqb) % 2;
+  const int l = g / (2 * nqb);
   const int K = M / 2, KZ = Z / 2;
   const double2* Vt = V + t * vstride;
   for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
@@ -1102,6 +1120,7 @@ __device__ void mv_mfma(const double* __restrict__ AT, int N, const double* X, d
 #include "jsd_solve.inc"
 
 #include "jsd_adjoint.inc"
+#include "jsd_adjoint_nl.inc"
 
 // ===========================================================================
 // energy: E = sum_c sum_l (w_l / 4) s^2/(Px Pz) sum_k |Herm(u_c)|^2
@@ -1181,6 +1200,8 @@ struct jsd_plan {
   double *YF1, *YF2;  // parity-block fragments of D and D^2 for k_ypar
   YparArgs yp1, yp2;  // their block geometry
   double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
+  int adjoint;         // the discrete-adjoint tables are present
+  double2* SS;         // adjoint of a nonlinear step: the three recomputed substep states
   double *DEO, *DOE, *D2E, *D2O, *RBP, *LIP, *POP, *CW, *RW, *VAP, *VINV;  // parity blocks of the solve
   double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
   int nE, nO;
@@ -1534,7 +1555,11 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     TRY(upload(p, &p->VINV, vinv.data(), vinv.size()));
   }
   p->DFT1 = p->D_R = p->D2_R = p->RB_R = p->LI_R = p->PO_R = nullptr;
-  if (d->kind == JSD_KIND_PERTURBATION && d->linearise) {
+  p->SS = nullptr;
+  const bool adj = (d->kind == JSD_KIND_PERTURBATION && d->linearise) ||
+                   (d->adjoint && (d->kind == JSD_KIND_NS || d->kind == JSD_KIND_PERTURBATION));
+  p->adjoint = adj ? 1 : 0;
+  if (adj) {
     // the discrete adjoint applies the transposes: row-major copies of the
     // tables (mv_mfma reads its operand transposed) and D^T fragments
     std::vector<double> DT(nn);
@@ -1548,7 +1573,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     TRY(upload(p, &p->PO_R, d->poisson, nk * nn));
   }
   p->VA = p->INFL = nullptr;
-  if (d->kind == JSD_KIND_PERTURBATION && d->linearise) {
+  if (adj) {
     TRY(upload(p, &p->VA, d->v_a, 3 * nk * N));
     TRY(upload(p, &p->INFL, d->infl_inv, 3 * nk * 4));
   }
@@ -1641,8 +1666,8 @@ static YparArgs ypar_args(const jsd_plan* p, const void* in, void* out, int orde
   G.Z = p->Z;
   G.comp_stride = (long)(2 * p->fs);
   const long ncol = (long)p->M * 2 * p->Z;
-  *nblk = (unsigned)((ncol + 63) / 64);
-  *lds = std::max(*lds, (size_t)std::max(G.ET * G.KIE, G.OT * G.KIO) * 64 * sizeof(double));
+  *nblk = (unsigned)((ncol + 31) / 32);
+  *lds = std::max(*lds, (size_t)0);
   return G;
 }
 
@@ -1791,13 +1816,14 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   prof_end(p, 2, st);
   HIPCHK(hipGetLastError());
   prof_begin(p, 3, st);
+  const dim3 grid_xf(8 * ((2 * nqb * N + 7) / 8));
   const bool xf_ct = ct && visit_x(p->fpx, [&](auto c) {
-    hipLaunchKernelGGL(k_fft_x_fwd_t<decltype(c)>, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H,
-                       (long)fs, p->fpx.tw, M, N, Z, ZT, p->s_fwd);
+    hipLaunchKernelGGL(k_fft_x_fwd_t<decltype(c)>, grid_xf, dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs,
+                       p->fpx.tw, M, N, Z, ZT, nqb, p->s_fwd);
   });
   if (!xf_ct)
-    hipLaunchKernelGGL(k_fft_x_fwd, dim3(nqb, N, 2), dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx,
-                       M, N, Z, ZT, p->s_fwd);
+    hipLaunchKernelGGL(k_fft_x_fwd, grid_xf, dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx, M, N, Z,
+                       ZT, nqb, p->s_fwd);
   prof_end(p, 3, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
@@ -2044,6 +2070,142 @@ static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipS
   return JSD_OK;
 }
 
+// Forward part of the nonlinear term at state U: D u_c into DU and the six
+// packed inverse x transforms into T (the inputs of the z pass).
+static int forward_inverse_x(jsd_plan* p, const double2* U, hipStream_t st) {
+  const size_t fs = p->fs;
+  const int M = p->M, N = p->N, Z = p->Z, ZT = p->ZT;
+  int rc = launch_ygemm_plain(p, U, p->DU, 3, 1, st);
+  if (rc) return rc;
+  const int nqb = (p->KZ + 1 + QB - 1) / QB;
+  const size_t ts = (size_t)p->d.ppx * N * ZT;
+  const size_t lds_x = (size_t)p->d.ppx * NSLOT * sizeof(double2);
+  XInvSet set;
+  for (int c = 0; c < 3; ++c) {
+    set.A[2 * c] = U + c * fs;
+    set.opA[2 * c] = OP_ID;
+    set.B[2 * c] = U + c * fs;
+    set.opB[2 * c] = OP_DX;
+    set.A[2 * c + 1] = p->DU + c * fs;
+    set.opA[2 * c + 1] = OP_ID;
+    set.B[2 * c + 1] = U + c * fs;
+    set.opB[2 * c + 1] = OP_DZ;
+  }
+  for (int t = 0; t < 6; ++t) set.T[t] = p->T + t * ts;
+  const int ntr = 6;
+  const dim3 grid_xi(8 * ((ntr * nqb * N + 7) / 8));
+  const bool ok = visit_x(p->fpx, [&](auto c) {
+    hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx.tw, p->kx,
+                       p->kz, M, N, Z, ZT, p->s_inv);
+  });
+  if (!ok)
+    hipLaunchKernelGGL(k_fft_x_inv, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx, p->kx, p->kz, M, N, Z, ZT,
+                       p->s_inv);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+// UB += (dH/du at Us)^T Hbar, Hbar in p->H (overwritten); the transpose of the
+// nonlinear term of the NS / nonlinear perturbation step.
+static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStream_t st) {
+  const jsd_plan_desc& d = p->d;
+  const size_t fs = p->fs;
+  const int M = p->M, N = p->N, Z = p->Z, ZT = p->ZT;
+  const int nqb = (p->KZ + 1 + QB - 1) / QB;
+  const size_t ts = (size_t)d.ppx * N * ZT;
+  const size_t lds_x = (size_t)d.ppx * NSLOT * sizeof(double2);
+  // (i) Hbar -> two packed rows of h_bar, into V
+  {
+    const dim3 grid(8 * ((2 * nqb * N + 7) / 8));
+    const bool ok = visit_x(p->fpx, [&](auto c) {
+      hipLaunchKernelGGL(k_adj_x_inv_t<decltype(c)>, grid, dim3(256), lds_x, st, p->H, (long)fs, p->V, (long)ts, nqb,
+                         p->fpx.tw, M, N, Z, ZT, p->s_fwd);
+    });
+    if (!ok) JSD_FAIL(JSD_ERR_ARG, "x transform length %d has no adjoint kernel", d.ppx);
+    HIPCHK(hipGetLastError());
+  }
+  // forward fields at the substep's state: DU, T
+  int rc = forward_inverse_x(p, Us, st);
+  if (rc) return rc;
+  // (ii) products adjoint, in place over T
+  {
+    const int prod = d.kind == JSD_KIND_NS ? PROD_NS : PROD_PERT;
+    const double clin = 1.0;
+    const dim3 grid(d.ppx, N);
+    const bool ok = visit_z(p->fpz, [&](auto c) {
+      using C = decltype(c);
+      const size_t lds = (size_t)C::P * 8 * sizeof(double2);
+      if (prod == PROD_NS)
+        hipLaunchKernelGGL((k_adj_z_t<C, PROD_NS>), grid, dim3(256), lds, st, p->T, (long)ts, p->V, (long)ts,
+                           p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+      else
+        hipLaunchKernelGGL((k_adj_z_t<C, PROD_PERT>), grid, dim3(256), lds, st, p->T, (long)ts, p->V, (long)ts,
+                           p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+    });
+    if (!ok) JSD_FAIL(JSD_ERR_ARG, "z transform length %d has no adjoint kernel", d.ppz);
+    HIPCHK(hipGetLastError());
+  }
+  // (iii) forward x transforms, Herm and derivative factors: odd rows, then even
+  for (int parity = 1; parity >= 0; --parity) {
+    const bool ok = visit_x(p->fpx, [&](auto c) {
+      hipLaunchKernelGGL(k_adj_x_fwd_t<decltype(c)>, dim3(nqb, N, 3), dim3(256), lds_x, st, p->T, (long)ts, parity, UB,
+                         p->DU, p->H, (long)fs, p->fpx.tw, p->kx, p->kz, M, N, Z, ZT, p->s_inv);
+    });
+    if (!ok) JSD_FAIL(JSD_ERR_ARG, "x transform length %d has no adjoint kernel", d.ppx);
+    HIPCHK(hipGetLastError());
+  }
+  // (iv) ubar_c += D^T Dubar_c
+  hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, ygemm_grid(p, 3), dim3(256), 0, st,
+                     ygemm_args(p, p->DFT1, p->DU, nullptr, nullptr, UB, nullptr, nullptr, nullptr, (long)(2 * fs)));
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s, bool has_carry, hipStream_t st) {
+  const Roles r = roles_of(p);
+  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));
+  HIPCHK(hipGetLastError());
+  int rc = nl_adjoint_terms(p, UB, Us, st);
+  if (rc) return rc;
+  std::swap(p->HV, p->HVo);
+  std::swap(p->HG, p->HGo);
+  std::swap(p->CONV, p->CONVo);
+  return JSD_OK;
+}
+
+extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, double dPdx, void* stream) {
+  if (!p || !state || !grad) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (p->d.kind == JSD_KIND_PERTURBATION && p->d.linearise) return jsd_step_vjp(p, grad, 1, stream);
+  if (p->d.kind == JSD_KIND_DUAL || !p->adjoint)
+    JSD_FAIL(JSD_ERR_STATE, "the nonlinear adjoint needs an NS or perturbation plan created with adjoint = 1");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t sb = 3 * p->fs * sizeof(double2);
+  if (!p->SS) {
+    int rc = dalloc(p, (void**)&p->SS, 3 * sb);
+    if (rc) return rc;
+  }
+  // substep states 0, 1, 2 of the step
+  double2* S0 = p->SS;
+  double2* S1 = p->SS + 3 * p->fs;
+  double2* S2 = p->SS + 6 * p->fs;
+  HIPCHK(hipMemcpyAsync(S0, state, sb, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(S1, state, sb, hipMemcpyDeviceToDevice, st));
+  int rc = substep(p, S1, nullptr, 0, dPdx, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(S2, S1, sb, hipMemcpyDeviceToDevice, st));
+  rc = substep(p, S2, nullptr, 1, dPdx, st);
+  if (rc) return rc;
+  const double2* Ss[3] = {S0, S1, S2};
+  for (int s = 2; s >= 0; --s) {
+    rc = adjoint_substep_nl(p, (double2*)grad, Ss[s], s, s < 2, st);
+    if (rc) return rc;
+  }
+  return JSD_OK;
+}
+
 extern "C" int jsd_step_vjp(jsd_plan* p, void* grad, int nsteps, void* stream) {
   if (!p || !grad || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   if (p->d.kind != JSD_KIND_PERTURBATION || !p->d.linearise || !p->LI_R)
@@ -2087,8 +2249,8 @@ struct OpYgemmAcc {
 struct OpYpar {
   typedef YparArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) {
-    extern __shared__ double sF[];
-    ypar_body(a, k.x, k.z, sF);
+    __shared__ double sO[YP_TMAX * 16 * 32];
+    ypar_body(a, k.x, k.z, sO);
   }
 };
 struct OpNlLin {

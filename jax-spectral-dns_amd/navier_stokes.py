@@ -73,6 +73,9 @@ class NavierStokesVelVort(Equation):
     def get_Re_tau(self) -> float:
         return self.Re_tau
 
+    def get_u_max_over_u_tau(self) -> float:
+        return self.u_max_over_u_tau
+
     def get_physical_domain(self) -> PhysicalDomain:
         return self.physical_domain
 

@@ -48,6 +48,7 @@ class StepPlan:
         ops: Optional[OperatorTables] = None,
         explicit_dt: Optional[float] = None,
         plan_re: Optional[float] = None,
+        adjoint: bool = False,
     ):
         if not torch.cuda.is_available():
             raise _lib.JsdError("no GPU visible: the channel-flow step runs only through the HIP library")
@@ -89,6 +90,8 @@ class StepPlan:
         desc.base_u = _ptr(base_u)
         desc.base_du = _ptr(base_du)
         desc.y_weights = _ptr(arr(g.y_weights))
+        desc.adjoint = int(bool(adjoint))
+        self.has_adjoint = bool(adjoint) or (kind == _lib.KIND_PERTURBATION and self.linearise)
         h = ctypes.c_void_p()
         _lib.check(lib.jsd_plan_create(ctypes.byref(desc), ctypes.byref(h)))
         self._h = h
@@ -100,9 +103,19 @@ class StepPlan:
         return not (self.kind == _lib.KIND_PERTURBATION and self.linearise)
 
     def vjp(self, grad: torch.Tensor, nsteps: int = 1) -> torch.Tensor:
-        """In place: adjoint after nsteps steps -> adjoint before them."""
+        """In place: adjoint after nsteps steps -> adjoint before them
+        (linearised perturbation plans)."""
         self._check_state(grad)
         _lib.check(_lib.load().jsd_step_vjp(self._h, _dev(grad), int(nsteps), _stream()))
+        return grad
+
+    def vjp_state(self, state: torch.Tensor, grad: torch.Tensor, dPdx: float = 0.0) -> torch.Tensor:
+        """In place: adjoint after one step -> adjoint before it, the step
+        taken from the forward state `state` (nonlinear plans built with
+        adjoint=True; linearised plans ignore `state`)."""
+        self._check_state(state)
+        self._check_state(grad)
+        _lib.check(_lib.load().jsd_step_vjp_state(self._h, _dev(state), _dev(grad), float(dPdx), _stream()))
         return grad
 
     @property

@@ -1,17 +1,21 @@
-"""Discrete adjoint of the linearised perturbation step behind
-``torch.autograd`` (the build's counterpart of reverse-mode AD through the
-reference's scan, jax.value_and_grad in optimiser.py:115-121).
+"""Discrete adjoint of the step behind ``torch.autograd`` (the build's
+counterpart of reverse-mode AD through the reference's scan,
+jax.value_and_grad in optimiser.py:115-121): the linearised perturbation
+step, and the NS / nonlinear perturbation step linearised about its forward
+state.
 
 ``run(u0, plan, nsteps, checkpoint_every)`` advances a complex128 state
 (given as its float64 real-pair view, shape (3, Nx, Ny, Nz, 2)) through
 nsteps RK3/CN steps on the GPU; its backward applies the hand-written
 transposed kernels (``jsd_step_vjp``).  The forward keeps a checkpoint every
 ``checkpoint_every`` steps and the backward walks the chunks in reverse,
-recomputing a chunk's states only when the adjoint needs them: the
-linearised step is linear, so its transpose needs no forward state and no
-recompute happens.  ``energy(u, plan)`` is the energy functional with its
-gradient from ``jsd_energy_grad``; ``gain_and_grad`` evaluates
-J = E(u_T)/E(u_0) and dJ/du_0.
+recomputing a chunk's step-start states from its checkpoint when the
+adjoint needs them (nonlinear steps: ``jsd_step_vjp_state`` then recomputes
+the three substep states of each step); the linearised step is linear, so
+its transpose needs no forward state and nothing is stored or recomputed.
+``energy(u, plan)`` is the energy functional with its gradient from
+``jsd_energy_grad``; ``gain_and_grad`` evaluates J = E(u_T)/E(u_0) and
+dJ/du_0.
 """
 
 from __future__ import annotations
@@ -37,7 +41,7 @@ def _energy_grad(plan: StepPlan, u: torch.Tensor, scale: float) -> torch.Tensor:
 
 class _Run(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int):  # type: ignore[override]
+    def forward(ctx, u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int, dPdx: float):  # type: ignore[override]
         u = torch.view_as_complex(u0r.contiguous()).clone()
         k = max(1, min(checkpoint_every, max(1, nsteps)))
         chunks = []
@@ -45,10 +49,11 @@ class _Run(torch.autograd.Function):
         while done < nsteps:
             c = min(k, nsteps - done)
             chunks.append((done, c, u.clone() if plan.adjoint_needs_state else None))
-            plan.step(u, 0.0, c)
+            plan.step(u, dPdx, c)
             done += c
         ctx.plan = plan
         ctx.chunks = chunks
+        ctx.dPdx = dPdx
         return torch.view_as_real(u).clone()
 
     @staticmethod
@@ -56,9 +61,19 @@ class _Run(torch.autograd.Function):
         plan: StepPlan = ctx.plan
         g = torch.view_as_complex(gr.contiguous()).clone()
         for start, count, state in reversed(ctx.chunks):
-            # a nonlinear adjoint would recompute the chunk's states from `state` here
-            _vjp(plan, g, count)
-        return torch.view_as_real(g), None, None, None
+            if state is None:
+                _vjp(plan, g, count)
+                continue
+            # recompute the chunk's step-start states from its checkpoint, then
+            # walk its steps backwards
+            states = [state]
+            x = state.clone()
+            for _ in range(count - 1):
+                plan.step(x, ctx.dPdx, 1)
+                states.append(x.clone())
+            for st in reversed(states):
+                plan.vjp_state(st, g, ctx.dPdx)
+        return torch.view_as_real(g), None, None, None, None
 
 
 class _Energy(torch.autograd.Function):
@@ -77,20 +92,25 @@ class _Energy(torch.autograd.Function):
         return torch.view_as_real(g), None
 
 
-def run(u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int = 50) -> torch.Tensor:
-    if plan.kind != _lib.KIND_PERTURBATION or not plan.linearise:
-        raise _lib.JsdError("the discrete adjoint is built for linearised perturbation plans")
-    return _Run.apply(u0r, plan, nsteps, checkpoint_every)
+def run(u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int = 50, dPdx: float = 0.0) -> torch.Tensor:
+    linear = plan.kind == _lib.KIND_PERTURBATION and plan.linearise
+    if not linear and not (plan.kind in (_lib.KIND_NS, _lib.KIND_PERTURBATION) and plan.has_adjoint):
+        raise _lib.JsdError(
+            "the discrete adjoint needs a linearised perturbation plan, or an NS / perturbation plan built with adjoint=True"
+        )
+    return _Run.apply(u0r, plan, nsteps, checkpoint_every, float(dPdx))
 
 
 def energy(ur: torch.Tensor, plan: StepPlan) -> torch.Tensor:
     return _Energy.apply(ur, plan)
 
 
-def gain_and_grad(u0: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int = 50) -> Tuple[float, torch.Tensor]:
+def gain_and_grad(
+    u0: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int = 50, dPdx: float = 0.0
+) -> Tuple[float, torch.Tensor]:
     """J = E(u_T) / E(u_0) and dJ/du_0 (complex128, real-pair convention:
     d/dRe + i d/dIm)."""
     x = torch.view_as_real(u0.contiguous()).clone().requires_grad_(True)
-    J = energy(run(x, plan, nsteps, checkpoint_every), plan) / energy(x, plan)
+    J = energy(run(x, plan, nsteps, checkpoint_every, dPdx), plan) / energy(x, plan)
     (gx,) = torch.autograd.grad(J, x)
     return float(J.detach()), torch.view_as_complex(gx.contiguous())

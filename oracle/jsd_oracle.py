@@ -803,3 +803,67 @@ def dual_gradient(fwd: StepParams, u0_hat: torch.Tensor, n_steps: int, linearise
     e0 = energy(dom, u0_hat)
     gain = energy(dom, uT) / e0
     return (u0_hat - v) / e0, gain, v
+
+
+# ---------------------------------------------------------------------------
+# gradient parameterisation (navier_stokes.py:1027-1203)
+# ---------------------------------------------------------------------------
+
+
+def clamped_cheb_basis(y: np.ndarray, M: Optional[int] = None) -> np.ndarray:
+    """Columns phi_n(y_j) = T_{n+4} - a T_{n+2} - b T_n, a = 2(n+2)/(n+1),
+    b = -(n+3)/(n+1), n < M = N-4: Chebyshev combinations with
+    phi(+-1) = phi'(+-1) = 0.  smooth_and_enforce_bc_vel_y.phi_n,
+    navier_stokes.py:1033-1048, evaluated at the grid as in :1054-1058."""
+    N = len(y)
+    M_ = N - 4 if M is None else M
+    A = np.zeros((N, M_))
+    for i in range(M_):
+        a = +(2 * (i + 2)) / (i + 1)
+        b = -(i + 3) / (i + 1)
+        arr = np.eye(4 + i + 1)[4 + i] - a * np.eye(4 + i + 1)[2 + i] - b * np.eye(4 + i + 1)[i]
+        A[:, i] = np.polynomial.chebyshev.chebval(y, arr)
+    return A
+
+
+def smooth_and_enforce_bc_vel_y(dom: Dom, v: torch.Tensor, M: Optional[int] = None) -> torch.Tensor:
+    """Least-squares projection of the y-profiles v[..., :] (last axis y)
+    onto the clamped basis.  navier_stokes.py:1027-1061 (jnp.linalg.lstsq)."""
+    A = torch.from_numpy(clamped_cheb_basis(dom.y, M)).to(v.dtype)
+    flat = v.reshape(-1, v.shape[-1]).T  # (N, batch)
+    coeffs = torch.linalg.lstsq(A, flat).solution
+    return (A @ coeffs).T.reshape(v.shape)
+
+
+def vort_yvel_to_vel(
+    dom: Dom,
+    vort: torch.Tensor,
+    vel_y: torch.Tensor,
+    vel_x_00: torch.Tensor,
+    vel_z_00: torch.Tensor,
+    two_d: bool = False,
+) -> torch.Tensor:
+    """(omega_y, v, u_00, w_00) -> (u, v, w), arrays (Nx, Ny, Nz): v smoothed
+    with the clamped projection, u and w from continuity and the definition
+    of omega_y per (kx, kz); the (0,0) pencil takes u_00, w_00 and the
+    smoothed v_00.  navier_stokes.py:1063-1203 (rk_00 :1087-1101,
+    rk_not_00 :1103-1130)."""
+    if two_d:
+        vort = torch.zeros_like(vel_y)
+        vel_z_00 = torch.zeros_like(vel_x_00)
+    vs = smooth_and_enforce_bc_vel_y(dom, vel_y.permute(0, 2, 1)).permute(0, 2, 1)
+    Dm = dom.t_("Dpow1", dom.D)
+    vy = torch.matmul(Dm, vs)  # diff along y: (Ny,Ny) @ (Nx, Ny, Nz)
+    kx = torch.from_numpy(dom.kx)[:, None, None]
+    kz = torch.from_numpy(dom.kz)[None, None, :]
+    mk2 = -(kx**2 + kz**2)
+    mk2 = torch.where(mk2 == 0, torch.ones_like(mk2), mk2)
+    u = (-1j * kx * vy + 1j * kz * vort) / mk2
+    w = (-1j * kz * vy - 1j * kx * vort) / mk2
+    if two_d:
+        w = torch.zeros_like(u)
+    u = u.clone()
+    w = w.clone()
+    u[0, :, 0] = vel_x_00.to(u.dtype)
+    w[0, :, 0] = vel_z_00.to(w.dtype)
+    return torch.stack([u, vs.to(u.dtype), w])

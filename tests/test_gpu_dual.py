@@ -70,3 +70,20 @@ def test_gradient_descent_step_increases_gain():
     gain1 = dual2.get_gain()
     print("gain %.6f -> %.6f" % (gain0, gain1))
     assert gain1 > gain0
+
+
+def test_steepest_adaptive_descent_increases_gain_monotonically():
+    """The optimisation driver (gradient_descent_solver.py:340-487): accepted
+    iterations increase the linear transient-growth gain at fixed energy."""
+    from jax_spectral_dns_amd.gradient_descent_solver import SteepestAdaptiveDescentSolver
+
+    shape, lx, lz, re, dt, n = (24, 65, 4), 2 * np.pi, 2 * np.pi, 3000.0, 0.01, 100
+    g, _ = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=0, energy=1e-6)
+    solver = SteepestAdaptiveDescentSolver(_nse(shape, lx, lz, re, dt, n * dt, True, u0), step_size=1e-2, max_iter=3)
+    vals = solver.optimise()
+    print("gains", vals)
+    assert all(b >= a for a, b in zip(vals, vals[1:]))
+    assert vals[-1] > vals[0]
+    e = solver.dual.forward_equation.plan().energy(solver.current_guess)
+    assert abs(e - solver.e_0) <= 1e-12 * solver.e_0

@@ -104,3 +104,32 @@ def test_gain_gradient_against_finite_differences():
         worst = max(worst, e)
         print("dir %2d fd %.12e ad %.12e rel %.2e" % (k, fd, ad, e))
     assert worst < 1e-8
+
+
+def test_parameter_gradient_against_oracle_autograd():
+    """dJ/d(omega_y, v, u_00, w_00) through vort_yvel_to_vel (the reference's
+    optimisation parameters) against autograd of the oracle, <= 1e-9."""
+    from jax_spectral_dns_amd.optimiser import gain_and_grad_parameters
+
+    shape = (24, 33, 16)
+    g, d = pair(shape, 2 * np.pi, np.pi)
+    re, dt, n = 3000.0, 0.01, 8
+    rng = np.random.default_rng(9)
+    sh = (g.nxm, g.ny, g.nzm)
+    vort = torch.from_numpy(rng.standard_normal(sh) + 1j * rng.standard_normal(sh)) * 1e-3
+    vy = torch.from_numpy(rng.standard_normal(sh) + 1j * rng.standard_normal(sh)) * 1e-3
+    ux = torch.from_numpy(rng.standard_normal(g.ny) + 0j) * 1e-3
+    wz = torch.from_numpy(rng.standard_normal(g.ny) + 0j) * 1e-3
+    plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=True)
+    J, grads = gain_and_grad_parameters(plan, [t.to(DEV) for t in (vort, vy, ux, wz)], n, checkpoint_every=3)
+    sp = O.make_perturbation(d, re, dt, True)
+    leaves = [torch.view_as_real(t.clone()).requires_grad_(True) for t in (vort, vy, ux, wz)]
+    u0 = O.vort_yvel_to_vel(d, *[torch.view_as_complex(x) for x in leaves])
+    Jr = O.energy(d, O.run(sp, u0, n)) / O.energy(d, u0)
+    ref = torch.autograd.grad(Jr, leaves)
+    print("J %.15f vs %.15f" % (J, float(Jr)))
+    assert abs(J - float(Jr)) / float(Jr) < 1e-11
+    for name, a, b in zip(("vort", "vel_y", "vel_x_00", "vel_z_00"), grads, ref):
+        err = rel(a, torch.view_as_complex(b))
+        print(name, "grad rel", err)
+        assert err < 1e-9, name

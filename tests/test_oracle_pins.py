@@ -343,3 +343,46 @@ def test_checkpointed_adjoint_identical():
     g2, gain2, v2 = O.dual_gradient(fwd, u0, 6, linearise=False, checkpoint=3)
     assert float(O.energy(d, v1 - v2)) < 1e-30
     assert float(gain1) == float(gain2)
+
+
+def test_parameterisation_pins():
+    """vort_yvel_to_vel (navier_stokes.py:1027-1203): the clamped projection
+    is idempotent, keeps profiles already in the clamped span, gives
+    v = v' = 0 at both walls; the velocity satisfies continuity and has the
+    prescribed omega_y and (0,0) profiles."""
+    shape = (8, 33, 8)
+    d = O.make_domain(shape, 2 * np.pi, np.pi)
+    rng = np.random.default_rng(0)
+    sh = (d.nxm, d.ny, d.nzm)
+    vort = torch.from_numpy(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    vy = torch.from_numpy(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    ux = torch.from_numpy(rng.standard_normal(d.ny) + 0j)
+    wz = torch.from_numpy(rng.standard_normal(d.ny) + 0j)
+    vel = O.vort_yvel_to_vel(d, vort, vy, ux, wz)
+    u, v, w = vel[0], vel[1], vel[2]
+    D = torch.from_numpy(d.D).to(torch.complex128)
+    dv = torch.matmul(D, v)
+    scale = float(v.abs().max())
+    assert float(v[:, 0, :].abs().max()) < 1e-12 * scale and float(v[:, -1, :].abs().max()) < 1e-12 * scale
+    assert float(dv[:, 0, :].abs().max()) < 1e-10 * float(dv.abs().max())
+    assert float(dv[:, -1, :].abs().max()) < 1e-10 * float(dv.abs().max())
+    # idempotent
+    v2 = O.smooth_and_enforce_bc_vel_y(d, v.permute(0, 2, 1)).permute(0, 2, 1)
+    assert float((v2 - v).abs().max()) < 1e-12 * scale
+    # a profile in the span is kept
+    A = torch.from_numpy(O.clamped_cheb_basis(d.y)).to(torch.complex128)
+    c = torch.from_numpy(rng.standard_normal(A.shape[1]) + 0j)
+    prof = A @ c
+    kept = O.smooth_and_enforce_bc_vel_y(d, prof[None, :])[0]
+    assert float((kept - prof).abs().max()) < 1e-12 * float(prof.abs().max())
+    # continuity i kx u + D v + i kz w = 0 and omega_y = i kz u - i kx w away from (0,0)
+    kx = torch.from_numpy(d.kx)[:, None, None]
+    kz = torch.from_numpy(d.kz)[None, None, :]
+    div = 1j * kx * u + dv + 1j * kz * w
+    div[0, :, 0] = 0
+    assert float(div.abs().max()) < 1e-11 * float(dv.abs().max())
+    om = 1j * kz * u - 1j * kx * w
+    mask = torch.ones_like(om)
+    mask[0, :, 0] = 0
+    assert float(((om - vort) * mask).abs().max()) < 1e-11 * float(vort.abs().max())
+    assert float((u[0, :, 0] - ux).abs().max()) == 0.0 and float((w[0, :, 0] - wz).abs().max()) == 0.0
