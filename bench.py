@@ -167,6 +167,52 @@ def batch_bench(nsteps: int) -> dict:
     }
 
 
+def adjoint_bench(nsteps: int) -> dict:
+    """One gain-gradient evaluation of the nonlinear perturbation step at
+    96 x 129 x 80 modes (Re 3000, Lx 2 pi, Lz pi, dt 0.005, E0 1e-2): forward
+    steps with a checkpoint every 10, then the discrete adjoint with
+    recompute; per-step costs and the wall time of the whole evaluation."""
+    import torch
+
+    from jax_spectral_dns_amd import KIND_PERTURBATION
+    from jax_spectral_dns_amd import domain_math as DM
+    from jax_spectral_dns_amd import initial
+    from jax_spectral_dns_amd.plan import StepPlan
+    from jax_spectral_dns_amd.vjp import gain_and_grad
+
+    g = DM.make_grid((96, 129, 80), 2 * np.pi, np.pi)
+    t0 = time.time()
+    plan = StepPlan(g, 3000.0, 0.005, KIND_PERTURBATION, linearise=False, adjoint=True)
+    plan.ops = None
+    t_build = time.time() - t0
+    u0 = torch.from_numpy(initial.perturbation(g, seed=0, energy=1e-2)).cuda()
+    u = u0.clone()
+    plan.step(u, 0.0, 2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.step(u, 0.0, nsteps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    gain_and_grad(u0, plan, 2, checkpoint_every=2)  # warm-up of the adjoint path
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, grad = gain_and_grad(u0, plan, nsteps, checkpoint_every=10)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    finite = bool(torch.isfinite(grad).all())
+    plan.close()
+    return {
+        "workload": "NavierStokesVelVortPerturbation nonlinear, 96x129x80 modes, Re 3000, Lx 2pi, Lz pi, dt 0.005, "
+        "E0 1e-2, gain gradient over %d steps, checkpoint every 10 with recompute" % nsteps,
+        "forward_ms_per_step": 1e3 * t_fwd / nsteps,
+        "gradient_eval_s": t_grad,
+        "adjoint_ms_per_step_incl_recompute": 1e3 * (t_grad - t_fwd) / nsteps,
+        "gain": J,
+        "grad_finite": finite,
+        "plan_build_s": round(t_build, 2),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +221,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-runs figure")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the nonlinear-gradient figure")
+    ap.add_argument("--adjoint-steps", type=int, default=40)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument(
         "--copy-probe",
@@ -335,6 +383,9 @@ def main() -> int:
         del u
         torch.cuda.empty_cache()
         out["batch"] = batch_bench(args.batch_steps)
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        out["nonlinear_gradient"] = adjoint_bench(args.adjoint_steps)
     print(json.dumps(out), flush=True)
     return 0
 
