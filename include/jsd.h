@@ -64,6 +64,16 @@ typedef struct jsd_plan_desc {
   const double* y_weights;  /* ny definite-integral weights */
   int adjoint;              /* nonzero: also build the tables of the discrete adjoint
                                (always built for linearised perturbation plans) */
+  const double* y_eig;      /* optional (NULL: the solve applies lhs_inv / poisson):
+                               eigen-decomposition of the interior D^2 (rows and
+                               columns 1..ny-2) on its even / odd halves
+                               (e_l = (x_l + x_l')/2, o_l = (x_l - x_l')/2,
+                               l' = ny-1-l; half sizes ne = ny/2-1+ny%2, no = ny/2-1):
+                               E_e (ne*ne), E_o (no*no), E_e^-1, E_o^-1 row-major,
+                               then lam_e (ne), lam_o (no).  The Helmholtz and Poisson
+                               inverses are then applied as E diag(.) E^-1 with
+                               beta_s |dt/Re| from dt and re_tau; checked against
+                               lhs_inv / poisson at plan creation */
 } jsd_plan_desc;
 
 /* Replaces NavierStokesVelVort.__init__ / assemble_matrices

@@ -48,6 +48,7 @@ class PlanDesc(ctypes.Structure):
         ("base_du", _dp),
         ("y_weights", _dp),
         ("adjoint", ctypes.c_int),
+        ("y_eig", _dp),
     ]
 
 

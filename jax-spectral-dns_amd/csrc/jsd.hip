@@ -651,32 +651,62 @@ __global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, int ntr, int n
   const int opA = set.opA[tr], opB = set.opB[tr];
   const bool sameAB = (A == B);  // (u_c, d_x u_c): one field read once
   const int K = M / 2, KZ = Z / 2;
-  for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
-    const int c = w % NSLOT;
-    const int pos = w / NSLOT;
+  // zero padding bins K+1 .. P-K-1 (3/2-rule grid beyond the stored modes)
+  for (int w = threadIdx.x; w < (P - M) * NSLOT; w += 256) {
+    const int c = w % NSLOT, n = K + 1 + w / NSLOT;
+    lds[dit_iperm<C, C::NS>(n) * NSLOT + c] = make_double2(0.0, 0.0);
+  }
+  // Hermitian pairs: item (i <= K, slot c) reads (i, kz_c) and its partner
+  // (-kx, -kz) = (M-i, slot c^QB) once and writes both bins, whose packed
+  // values are Herm(a) + i Herm(b) and its conjugate-pair counterpart.  All
+  // loads of a thread are issued before any is used.
+  constexpr int IT = ((P + 2) / 3 * NSLOT + 255) / 256;  // K + 1 <= (P + 2) / 3
+  const int nitem = (K + 1) * NSLOT;
+  double2 ra[IT], rap[IT], rb[IT], rbp[IT];
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int w = threadIdx.x + 256 * k;
+    const int c = w % NSLOT, i = w / NSLOT;
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
-    double2 val = make_double2(0.0, 0.0);
-    const int n = dit_perm<C, C::NS>(pos);  // bin index
-    int i = -1;
-    if (n <= K) i = n;
-    else if (n >= P - K) i = n - P + M;
-    if (ok && i >= 0) {
-      const int pi = (M - i) % M;
-      const int pj = (Z - j) % Z;
+    ra[k] = rap[k] = rb[k] = rbp[k] = make_double2(0.0, 0.0);
+    if (w < nitem && ok && !(i == 0 && c >= QB)) {
+      const int pi = (M - i) % M, pj = (Z - j) % Z;
       const size_t e = ((size_t)i * N + l) * Z + j;
       const size_t ep = ((size_t)pi * N + l) * Z + pj;
-      const double2 ra = A[e], rap = A[ep];
-      const double2 rb = sameAB ? ra : B[e], rbp = sameAB ? rap : B[ep];
-      const double2 a = apply_op(opA, ra, kx[i], kz[j]);
-      const double2 ap = apply_op(opA, rap, kx[pi], kz[pj]);
-      const double2 b = apply_op(opB, rb, kx[i], kz[j]);
-      const double2 bp = apply_op(opB, rbp, kx[pi], kz[pj]);
-      const double2 ha = cherm(a, ap);
-      const double2 hb = cherm(b, bp);
-      val = make_double2((ha.x - hb.y) * s_inv, (ha.y + hb.x) * s_inv);
+      ra[k] = A[e];
+      rap[k] = A[ep];
+      if (!sameAB) {
+        rb[k] = B[e];
+        rbp[k] = B[ep];
+      }
     }
-    lds[pos * NSLOT + c] = val;
+  }
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int w = threadIdx.x + 256 * k;
+    const int c = w % NSLOT, i = w / NSLOT;
+    if (w >= nitem) continue;
+    bool ok;
+    const int j = slot_col(c, q0, Z, KZ, &ok);
+    if (i == 0 && c >= QB && ok) continue;  // written by its partner item (0, c - QB)
+    const int pi = (M - i) % M, pj = (Z - j) % Z;
+    const int cp = slot_partner(c, q0);
+    double2 val = make_double2(0.0, 0.0), valp = val;
+    if (ok) {
+      const double2 b0 = sameAB ? ra[k] : rb[k], bp0 = sameAB ? rap[k] : rbp[k];
+      const double2 a = apply_op(opA, ra[k], kx[i], kz[j]);
+      const double2 ap = apply_op(opA, rap[k], kx[pi], kz[pj]);
+      const double2 b = apply_op(opB, b0, kx[i], kz[j]);
+      const double2 bp = apply_op(opB, bp0, kx[pi], kz[pj]);
+      const double2 ha = cherm(a, ap), hap = cherm(ap, a);
+      const double2 hb = cherm(b, bp), hbp = cherm(bp, b);
+      val = make_double2((ha.x - hb.y) * s_inv, (ha.y + hb.x) * s_inv);
+      valp = make_double2((hap.x - hbp.y) * s_inv, (hap.y + hbp.x) * s_inv);
+    }
+    const int n = i, np_ = pi <= K ? pi : P - (M - pi);  // bins of kx index i and M - i
+    lds[dit_iperm<C, C::NS>(n) * NSLOT + c] = val;
+    if (!(pi == i && cp == c)) lds[dit_iperm<C, C::NS>(np_) * NSLOT + cp] = valp;
   }
   __syncthreads();
   fft_lds_ct<C, NSLOT, NSLOT, 1, 256, +1>(lds, tw);
@@ -882,13 +912,22 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd_t(const double2* __restrict__
   const int l = g / (2 * nqb);
   const int K = M / 2, KZ = Z / 2;
   const double2* Vt = V + t * vstride;
-  for (int w = threadIdx.x; w < P * NSLOT; w += 256) {
-    const int c = w % NSLOT;
-    const int pos = w / NSLOT;
+  // all loads of a thread are issued before the LDS stores
+  constexpr int IT = (P * NSLOT + 255) / 256;
+  double2 vv[IT];
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int w = threadIdx.x + 256 * k;
+    const int c = w % NSLOT, pos = w / NSLOT;
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
-    const int xi = dit_perm<C, C::NS>(pos);
-    lds[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * ZT + j] : make_double2(0.0, 0.0);
+    vv[k] = make_double2(0.0, 0.0);
+    if (w < P * NSLOT && ok) vv[k] = Vt[((size_t)dit_perm<C, C::NS>(pos) * N + l) * ZT + j];
+  }
+#pragma unroll
+  for (int k = 0; k < IT; ++k) {
+    const int w = threadIdx.x + 256 * k;
+    if (w < P * NSLOT) lds[w] = vv[k];  // w = pos * NSLOT + c
   }
   __syncthreads();
   fft_lds_ct<C, NSLOT, NSLOT, 1, 256, -1>(lds, tw);
@@ -1203,6 +1242,10 @@ struct jsd_plan {
   int adjoint;         // the discrete-adjoint tables are present
   double2* SS;         // adjoint of a nonlinear step: the three recomputed substep states
   double *DEO, *DOE, *D2E, *D2O, *RBP, *LIP, *POP, *CW, *RW, *VAP, *VINV;  // parity blocks of the solve
+  double *EVF, *EIF, *LAM;  // eigen-form solve: E, E^-1 fragments [even, odd], eigenvalues [2][PH]
+  double *FF, *GF, *CWE, *RWE;  // E^-1 D and D E fragments, wall columns / rows in eigen coordinates
+  double c_impl[3];         // beta_s dt / Re of the Helmholtz operators
+  double *EH_R, *EG_R, *LAMR;  // adjoint eigen form: full-row E and E^-1 (row-major), eigenvalue per row
   double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
   int nE, nO;
   int *permx, *permz, *ipermx, *ipermz;
@@ -1364,6 +1407,222 @@ static void parity_frag(const double* A, int N, int sgn, int qi, int qo, double*
   parity_tile(blk.data(), frag);
 }
 
+static const double RK_BET[3] = {37.0 / 160.0, 5.0 / 24.0, 1.0 / 6.0};
+
+// Eigen-form solve tables.  The interior block of every implicit y operator
+// is a function of the interior D^2, which on each parity half is
+// E diag(lam) E^-1 (host precompute, desc->y_eig): the Helmholtz inverse is
+// E diag(1 / ((1 + c k^2) - c lam)) E^-1 with c = beta_s |dt / Re| and the
+// Poisson inverse E diag(1 / (lam - k^2)) E^-1.  The kernel then needs the
+// two shared blocks instead of one lhs_inv and one poisson block per |k|.
+// The factorisation is checked against the lhs_inv / poisson tables on a
+// sample of wavenumbers and every substep before it replaces them.
+static int eigen_tables(jsd_plan* p, const jsd_plan_desc* d, int N, size_t nk, const double* deo, const double* doe,
+                        const double* cw, const double* rw) {
+  const int nE = p->nE, nO = p->nO;
+  const double* f = d->y_eig;
+  const double* Ee = f;
+  const double* Eo = Ee + (size_t)nE * nE;
+  const double* Xe = Eo + (size_t)nO * nO;
+  const double* Xo = Xe + (size_t)nE * nE;
+  const double* le = Xo + (size_t)nO * nO;
+  const double* lo = le + nE;
+  std::vector<double> blk((size_t)PH * PH), frag(2 * (size_t)PFRAG), ifrag(2 * (size_t)PFRAG), lam(2 * PH, -1.0);
+  auto pad = [&](const double* src, int n, double* out) {
+    std::fill(blk.begin(), blk.end(), 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) blk[(size_t)i * PH + j] = src[(size_t)i * n + j];
+    parity_tile(blk.data(), out);
+  };
+  pad(Ee, nE, frag.data());
+  pad(Eo, nO, frag.data() + PFRAG);
+  pad(Xe, nE, ifrag.data());
+  pad(Xo, nO, ifrag.data() + PFRAG);
+  for (int h = 0; h < nE; ++h) lam[h] = le[h];
+  for (int h = 0; h < nO; ++h) lam[PH + h] = lo[h];
+  for (int h = 0; h < nE + nO; ++h) {
+    const double l = h < nE ? le[h] : lo[h - nE];
+    if (!(l < 0.0)) JSD_FAIL(JSD_ERR_ARG, "y_eig: eigenvalue %d of the interior D^2 is %g (must be negative)", h, l);
+  }
+  for (int s = 0; s < 3; ++s) p->c_impl[s] = RK_BET[s] * fabs(d->dt) / fabs(d->re_tau);
+  // check: table application vs eigen form on a fixed pseudo-random vector
+  std::vector<double> x(N, 0.0), yt(N), vt(N), ye(N), ve(N), he(PH), ho(PH), z(PH);
+  uint64_t st = 0x9e3779b97f4a7c15ull;
+  for (int l = 1; l < N - 1; ++l) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    x[l] = (double)(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+  }
+  for (int h = 0; h < nE; ++h) {
+    const int l = h + 1, lm = N - 1 - l;
+    he[h] = l == lm ? x[l] : 0.5 * (x[l] + x[lm]);
+    if (h < nO) ho[h] = 0.5 * (x[l] - x[lm]);
+  }
+  const size_t nn = (size_t)N * N;
+  for (size_t k = 0; k < nk; ++k) {
+    if (k % 13 != 0 && k != nk - 1) continue;
+    const int a = (int)(k / p->Kb), b = (int)(k % p->Kb);
+    const double k2 = d->kx[a] * d->kx[a] + d->kz[b] * d->kz[b];
+    for (int s = 0; s < 3; ++s) {
+      const double* L = d->lhs_inv + ((size_t)s * nk + k) * nn;
+      const double* P = d->poisson + k * nn;
+      for (int i = 0; i < N; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < N; ++j) acc += L[(size_t)i * N + j] * x[j];
+        yt[i] = acc;
+      }
+      for (int i = 0; i < N; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < N; ++j) acc += P[(size_t)i * N + j] * yt[j];
+        vt[i] = acc;
+      }
+      std::fill(ye.begin(), ye.end(), 0.0);
+      std::fill(ve.begin(), ve.end(), 0.0);
+      const double c = p->c_impl[s];
+      for (int q = 0; q < 2; ++q) {
+        const int n = q ? nO : nE;
+        const double* E = q ? Eo : Ee;
+        const double* X = q ? Xo : Xe;
+        const double* lq = q ? lo : le;
+        const double* in = q ? ho.data() : he.data();
+        for (int i = 0; i < n; ++i) {
+          double acc = 0.0;
+          for (int j = 0; j < n; ++j) acc += X[(size_t)i * n + j] * in[j];
+          z[i] = acc;
+        }
+        for (int i = 0; i < n; ++i) {
+          double om = 0.0, v = 0.0;
+          for (int j = 0; j < n; ++j) {
+            const double dl = 1.0 / ((1.0 + c * k2) - c * lq[j]);
+            om += E[(size_t)i * n + j] * dl * z[j];
+            v += E[(size_t)i * n + j] * (dl / (lq[j] - k2)) * z[j];
+          }
+          const int l = i + 1, lm = N - 1 - l;
+          const double sg = q ? -1.0 : 1.0;
+          ye[l] += om;
+          ve[l] += v;
+          if (lm != l) {
+            ye[lm] += sg * om;
+            ve[lm] += sg * v;
+          }
+        }
+      }
+      double ey = 0.0, ny = 0.0, ev = 0.0, nv = 0.0;
+      for (int l = 1; l < N - 1; ++l) {
+        ey += (ye[l] - yt[l]) * (ye[l] - yt[l]);
+        ny += yt[l] * yt[l];
+        ev += (ve[l] - vt[l]) * (ve[l] - vt[l]);
+        nv += vt[l] * vt[l];
+      }
+      const double ry = sqrt(ey / ny), rv = sqrt(ev / nv);
+      if (!(ry < 1e-9) || !(rv < 1e-9))
+        JSD_FAIL(JSD_ERR_ARG,
+                 "y_eig does not reproduce the lhs_inv / poisson tables at |k| index %zu, substep %d "
+                 "(rel. errors %.3g, %.3g): tables not built from this D, dt and Re",
+                 k, s, ry, rv);
+    }
+  }
+  // composite blocks of the eigen-coordinate solve (extended-precision sums):
+  // F = E^-1 D (even coordinates <- odd rows: X_e DEO; odd <- even: X_o DOE),
+  // G = D E (even rows <- odd coordinates: DEO E_o; odd <- even: DOE E_e),
+  // the wall columns CW and wall rows RW of D, D^2 carried to the coordinates
+  std::vector<double> ff(2 * (size_t)PFRAG), gf(2 * (size_t)PFRAG), cwe(4 * PH, 0.0), rwe(2 * PH, 0.0);
+  auto prod = [&](const double* A, int lda, int ar, int ak, const double* B, int ldb, int bc, double* out) {
+    std::fill(blk.begin(), blk.end(), 0.0);
+    for (int i = 0; i < ar; ++i)
+      for (int j = 0; j < bc; ++j) {
+        long double acc = 0.0L;
+        for (int k = 0; k < ak; ++k) acc += (long double)A[(size_t)i * lda + k] * B[(size_t)k * ldb + j];
+        blk[(size_t)i * PH + j] = (double)acc;
+      }
+    parity_tile(blk.data(), out);
+  };
+  prod(Xe, nE, nE, nE, deo, PH, nO, ff.data());           // X_e (nE x nE) . DEO (nE x nO)
+  prod(Xo, nO, nO, nO, doe, PH, nE, ff.data() + PFRAG);   // X_o (nO x nO) . DOE (nO x nE)
+  prod(deo, PH, nE, nO, Eo, nO, nO, gf.data());           // DEO (nE x nO) . E_o (nO x nO)
+  prod(doe, PH, nO, nE, Ee, nE, nE, gf.data() + PFRAG);   // DOE (nO x nE) . E_e (nE x nE)
+  for (int q = 0; q < 2; ++q) {
+    const int n = q ? nO : nE;
+    const double* X = q ? Xo : Xe;
+    for (int w = 0; w < 2; ++w) {
+      const double* c = cw + (size_t)(2 * w + q) * PH;  // CW[q] (D), CW[2 + q] (D^2)
+      for (int i = 0; i < n; ++i) {
+        long double acc = 0.0L;
+        for (int k = 0; k < n; ++k) acc += (long double)X[(size_t)i * n + k] * c[k];
+        cwe[(size_t)(2 * w + q) * PH + i] = (double)acc;
+      }
+    }
+  }
+  for (int c = 0; c < nO; ++c) {
+    long double acc = 0.0L;
+    for (int h = 0; h < nO; ++h) acc += (long double)rw[h] * Eo[(size_t)h * nO + c];
+    rwe[c] = (double)acc;
+  }
+  for (int c = 0; c < nE; ++c) {
+    long double acc = 0.0L;
+    for (int h = 0; h < nE; ++h) acc += (long double)rw[PH + h] * Ee[(size_t)h * nE + c];
+    rwe[PH + c] = (double)acc;
+  }
+  int rc = upload(p, &p->EVF, frag.data(), frag.size());
+  if (!rc) rc = upload(p, &p->EIF, ifrag.data(), ifrag.size());
+  if (!rc) rc = upload(p, &p->LAM, lam.data(), lam.size());
+  if (!rc) rc = upload(p, &p->FF, ff.data(), ff.size());
+  if (!rc) rc = upload(p, &p->GF, gf.data(), gf.size());
+  if (!rc) rc = upload(p, &p->CWE, cwe.data(), cwe.size());
+  if (!rc) rc = upload(p, &p->RWE, rwe.data(), rwe.size());
+  return rc;
+}
+
+// The eigen form on full rows, for the adjoint solve (which works on full
+// rows with row-major tables applied transposed): coordinate c of the
+// interior (c < nE: even eigenvector c, else odd eigenvector c - nE) sits in
+// row 1 + c, walls zero.  H maps coordinates to rows (H[l][1+c]: the
+// eigenvector spread over l and its mirror, with the odd sign), G maps rows
+// to coordinates (E^-1 applied to the parity halves e = (x_l + x_l')/2,
+// o = (x_l - x_l')/2), so the interior Helmholtz inverse is H diag(dl) G.
+static int eigen_tables_full(jsd_plan* p, const jsd_plan_desc* d, int N) {
+  const int nE = p->nE, nO = p->nO;
+  const double* Ee = d->y_eig;
+  const double* Eo = Ee + (size_t)nE * nE;
+  const double* Xe = Eo + (size_t)nO * nO;
+  const double* Xo = Xe + (size_t)nE * nE;
+  const double* le = Xo + (size_t)nO * nO;
+  const double* lo = le + nE;
+  const size_t nn = (size_t)N * N;
+  std::vector<double> H(nn, 0.0), G(nn, 0.0), lam(N, -1.0);
+  for (int h = 0; h < nE; ++h) {
+    const int l = h + 1, lm = N - 1 - l;
+    const bool centre = l == lm;
+    for (int c = 0; c < nE; ++c) {
+      H[(size_t)l * N + 1 + c] = Ee[(size_t)h * nE + c];
+      if (!centre) H[(size_t)lm * N + 1 + c] = Ee[(size_t)h * nE + c];
+      const double x = Xe[(size_t)c * nE + h];
+      if (centre) {
+        G[(size_t)(1 + c) * N + l] = x;
+      } else {
+        G[(size_t)(1 + c) * N + l] = 0.5 * x;
+        G[(size_t)(1 + c) * N + lm] = 0.5 * x;
+      }
+    }
+  }
+  for (int h = 0; h < nO; ++h) {
+    const int l = h + 1, lm = N - 1 - l;
+    for (int c = 0; c < nO; ++c) {
+      const int r = 1 + nE + c;
+      H[(size_t)l * N + r] = Eo[(size_t)h * nO + c];
+      H[(size_t)lm * N + r] = -Eo[(size_t)h * nO + c];
+      const double x = Xo[(size_t)c * nO + h];
+      G[(size_t)r * N + l] = 0.5 * x;
+      G[(size_t)r * N + lm] = -0.5 * x;
+    }
+  }
+  for (int c = 0; c < nE; ++c) lam[1 + c] = le[c];
+  for (int c = 0; c < nO; ++c) lam[1 + nE + c] = lo[c];
+  int rc = upload(p, &p->EH_R, H.data(), nn);
+  if (!rc) rc = upload(p, &p->EG_R, G.data(), nn);
+  if (!rc) rc = upload(p, &p->LAMR, lam.data(), (size_t)N);
+  return rc;
+}
+
 // Fragments of the full-row parity blocks of A (walls included: half index h
 // is row h and its mirror N-1-h), sgn = -1 for D (parities swap), +1 for D^2.
 static void ypar_frags(const double* A, int N, int sgn, std::vector<double>& F, YparArgs& G) {
@@ -1493,7 +1752,10 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
       }
       TRY(upload(p, &p->RBP, rb.data(), rb.size()));
     }
-    {
+    p->EVF = p->EIF = p->LAM = nullptr;
+    p->FF = p->GF = p->CWE = p->RWE = nullptr;
+    p->LIP = p->POP = nullptr;
+    if (!d->y_eig) {
       std::vector<double> li(3 * nk * 2 * (size_t)PFRAG), po(nk * 2 * (size_t)PFRAG);
       for (size_t k = 0; k < 3 * nk; ++k) {
         parity_frag(d->lhs_inv + k * nn, N, +1, 0, 0, li.data() + k * 2 * PFRAG);
@@ -1526,6 +1788,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     std::vector<double> deo((size_t)PH * PH), doe((size_t)PH * PH);
     parity_dense(d->D, N, -1, 1, 0, deo.data());
     parity_dense(d->D, N, -1, 0, 1, doe.data());
+    if (d->y_eig) TRY(eigen_tables(p, d, N, nk, deo.data(), doe.data(), cw.data(), rw.data()));
     std::vector<double> vap(3 * nk * 4 * (size_t)PH, 0.0), vinv(3 * nk * 2);
     for (size_t k = 0; k < 3 * nk; ++k) {
       const double* va = d->v_a + k * N;
@@ -1555,6 +1818,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     TRY(upload(p, &p->VINV, vinv.data(), vinv.size()));
   }
   p->DFT1 = p->D_R = p->D2_R = p->RB_R = p->LI_R = p->PO_R = nullptr;
+  p->EH_R = p->EG_R = p->LAMR = nullptr;
   p->SS = nullptr;
   const bool adj = (d->kind == JSD_KIND_PERTURBATION && d->linearise) ||
                    (d->adjoint && (d->kind == JSD_KIND_NS || d->kind == JSD_KIND_PERTURBATION));
@@ -1569,8 +1833,12 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     TRY(upload(p, &p->D_R, d->D, nn));
     TRY(upload(p, &p->D2_R, D2.data(), nn));
     TRY(upload(p, &p->RB_R, d->rhs_base, 3 * nn));
-    TRY(upload(p, &p->LI_R, d->lhs_inv, 3 * nk * nn));
-    TRY(upload(p, &p->PO_R, d->poisson, nk * nn));
+    if (p->EVF) {
+      TRY(eigen_tables_full(p, d, N));
+    } else {
+      TRY(upload(p, &p->LI_R, d->lhs_inv, 3 * nk * nn));
+      TRY(upload(p, &p->PO_R, d->poisson, nk * nn));
+    }
   }
   p->VA = p->INFL = nullptr;
   if (adj) {
@@ -1860,8 +2128,16 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
   S.D2O = p->D2O;
   S.RBE = p->RBP + (size_t)s * 2 * PFRAG;
   S.RBO = S.RBE + PFRAG;
-  S.LIP = p->LIP + (size_t)s * nk * 2 * PFRAG;
+  S.LIP = p->LIP ? p->LIP + (size_t)s * nk * 2 * PFRAG : nullptr;
   S.POP = p->POP;
+  S.EVF = p->EVF;
+  S.EIF = p->EIF;
+  S.FF = p->FF;
+  S.GF = p->GF;
+  S.CWE = p->CWE;
+  S.RWE = p->RWE;
+  S.LAM = p->LAM;
+  S.c_impl = p->EVF ? p->c_impl[s] : 0.0;
   S.CW = p->CW;
   S.RW = p->RW;
   S.VAP = p->VAP + (size_t)s * nk * 4 * PH;
@@ -1891,7 +2167,7 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
 static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st) {
   int rc = nonlinear_H(p, U, UF, st);
   if (rc) return rc;
-  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  const int nblk = ((solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
   prof_begin(p, 5, st);
   hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, solve_args(p, U, s, dPdx, roles_of(p)));
   prof_end(p, 5, st);
@@ -2020,8 +2296,12 @@ static SolveTArgs solve_T_args(const jsd_plan* p, double2* UB, int s, bool has_c
   S.D_R = p->D_R;
   S.D2_R = p->D2_R;
   S.RB_R = p->RB_R + s * nn;
-  S.LI_R = p->LI_R + (size_t)s * nk * nn;
+  S.LI_R = p->LI_R ? p->LI_R + (size_t)s * nk * nn : nullptr;
   S.PO_R = p->PO_R;
+  S.EH_R = p->EH_R;
+  S.EG_R = p->EG_R;
+  S.LAMR = p->LAMR;
+  S.c_impl = p->EH_R ? p->c_impl[s] : 0.0;
   S.VA = p->VA + (size_t)s * nk * p->N;
   S.INFL = p->INFL + (size_t)s * nk * 4;
   S.RNS = p->RNS + (size_t)s * 4 * nn;
@@ -2208,7 +2488,7 @@ extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, do
 
 extern "C" int jsd_step_vjp(jsd_plan* p, void* grad, int nsteps, void* stream) {
   if (!p || !grad || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
-  if (p->d.kind != JSD_KIND_PERTURBATION || !p->d.linearise || !p->LI_R)
+  if (p->d.kind != JSD_KIND_PERTURBATION || !p->d.linearise || !p->adjoint)
     JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint is built for linearised perturbation plans");
   hipStream_t st = (hipStream_t)stream;
   for (int n = 0; n < nsteps; ++n)
@@ -2263,7 +2543,12 @@ struct OpNlLinT {
 };
 struct OpSolve {
   typedef SolveArgs Args;
-  static __device__ __forceinline__ void run(const Args& a, Blk k) { solve_body(a, k.x); }
+  static __device__ __forceinline__ void run(const Args& a, Blk k) {
+    if (a.EVF)
+      solve_body_eig(a, k.x);
+    else
+      solve_body(a, k.x);
+  }
 };
 struct OpSolveT {
   typedef SolveTArgs Args;
@@ -2453,7 +2738,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_STATE, "run %d: batches hold linearised perturbation or dual plans only", r);
     }
-    if (b->vjp && (p->d.kind != JSD_KIND_PERTURBATION || !p->LI_R)) {
+    if (b->vjp && (p->d.kind != JSD_KIND_PERTURBATION || !p->adjoint)) {
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_STATE, "run %d: the adjoint is built for linearised perturbation plans", r);
     }
@@ -2485,7 +2770,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       yg.push_back(ygemm_grid(p, 1));
       na.push_back(nl_lin_args(p, U));
       ng.push_back(dim3((unsigned)((p->fs + 255) / 256)));
-      sg.push_back(dim3(p->Ka * p->Kb));
+      sg.push_back(dim3(solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb)));
     }
     BTRY(make_stage(b, b->yg, ya, yp));
     b->yg.lds = ylds;

@@ -185,6 +185,85 @@ class OperatorTables:
     rhs_base: np.ndarray  # (3, Ny, Ny) hom_dir(I + alpha_s dt D2 / Re)
     rhs_ns: np.ndarray  # (3, 2Ny, 2Ny)
     lhs_inv_ns: np.ndarray  # (3, 2Ny, 2Ny)
+    y_eig: np.ndarray  # interior_eigen(D^2), flat
+
+
+def parity_blocks(A: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Even and odd blocks of the interior rows/columns of a centro-symmetric
+    matrix (A[N-1-i][N-1-j] = A[i][j], after symmetrising (A + J A J)/2).
+
+    With e_l = (x_l + x_l')/2, o_l = (x_l - x_l')/2 (l' = N-1-l) on the
+    interior rows l = 1..N-2, A maps e -> e by the block
+    B_e[h][g] = A[h+1][g+1] + A[h+1][N-2-g] (the centre column once) and
+    o -> o by B_o[h][g] = A[h+1][g+1] - A[h+1][N-2-g]; half sizes
+    nE = N//2 - 1 + N%2, nO = N//2 - 1.
+    """
+    n = A.shape[0]
+    S = 0.5 * (A + A[::-1, ::-1])
+    ne, no = n // 2 - 1 + (n & 1), n // 2 - 1
+    rows = np.arange(1, ne + 1)
+    cols = np.arange(1, ne + 1)
+    mirror = n - 1 - cols
+    Be = S[np.ix_(rows, cols)] + np.where(cols == mirror, 0.0, 1.0)[None, :] * S[np.ix_(rows, mirror)]
+    ro = np.arange(1, no + 1)
+    Bo = S[np.ix_(ro, ro)] - S[np.ix_(ro, n - 1 - ro)]
+    return Be, Bo
+
+
+def _refined_eig(B: np.ndarray, sweeps: int = 3) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Real eigen-decomposition B = E diag(lam) E^-1 of a block of the interior
+    Chebyshev D^2 (real, distinct, negative eigenvalues).  LAPACK's pairs are
+    refined by Newton steps on the bordered system
+    [[B - lam I, -e], [e^T, 0]] [de; dlam] = [-(B e - lam e); 0] with the
+    residual in extended precision, and E^-1 by Newton-Schulz steps
+    X <- X + X (I - E X), so that E, E^-1 and lam are accurate to rounding."""
+    n = B.shape[0]
+    lam, E = np.linalg.eig(B)
+    if np.abs(lam.imag).max() > 0 or np.abs(E.imag).max() > 0:
+        raise ValueError("interior D^2 has complex eigenvalues")
+    lam, E = lam.real, E.real
+    order = np.argsort(-lam)
+    lam, E = lam[order], E[:, order]
+    Bl = B.astype(np.longdouble)
+    El = E.astype(np.longdouble)
+    lal = lam.astype(np.longdouble)
+    M = np.zeros((n + 1, n + 1))
+    for j in range(n):
+        e = El[:, j] / np.sqrt((El[:, j] ** 2).sum())
+        for _ in range(sweeps):
+            r = Bl @ e - lal[j] * e
+            M[:n, :n] = B - float(lal[j]) * np.eye(n)
+            M[:n, n] = -e.astype(np.float64)
+            M[n, :n] = e.astype(np.float64)
+            M[n, n] = 0.0
+            rhs = np.concatenate([-r.astype(np.float64), [0.0]])
+            d = np.linalg.solve(M, rhs)
+            e = e + d[:n].astype(np.longdouble)
+            lal[j] = lal[j] + np.longdouble(d[n])
+        El[:, j] = e
+    E = El.astype(np.float64)
+    X = np.linalg.inv(E).astype(np.longdouble)
+    El = E.astype(np.longdouble)
+    eye = np.eye(n, dtype=np.longdouble)
+    for _ in range(2):
+        X = X + X @ (eye - El @ X)
+    return E, X.astype(np.float64), lal.astype(np.float64)
+
+
+def interior_eigen(D2: np.ndarray) -> np.ndarray:
+    """Eigen-decompositions of the even and odd interior blocks of D^2
+    (parity_blocks).  The interior block of every implicit y operator of the
+    step is a function of it: the Helmholtz inverse
+    (hom_dir(I - beta_s dt L(k)))^-1 restricted to the interior is
+    E diag(1 / (1 + c k^2 - c lam)) E^-1 with c = beta_s dt / Re, and the
+    Poisson inverse E diag(1 / (lam - k^2)) E^-1 (navier_stokes.py:920-1022),
+    so one shared pair (E, E^-1) per parity replaces the per-|k| inverse
+    tables on the device.  Flat layout: E_e (nE x nE), E_o (nO x nO),
+    E_e^-1, E_o^-1 (row-major), lam_e (nE), lam_o (nO)."""
+    Be, Bo = parity_blocks(D2)
+    Ee, Xe, le = _refined_eig(Be)
+    Eo, Xo, lo = _refined_eig(Bo)
+    return np.concatenate([Ee.ravel(), Eo.ravel(), Xe.ravel(), Xo.ravel(), le, lo])
 
 
 def build_operators(g: Grid, re_tau: float, dt: float) -> OperatorTables:
@@ -247,6 +326,7 @@ def build_operators(g: Grid, re_tau: float, dt: float) -> OperatorTables:
         rhs_base=rhs_base,
         rhs_ns=rhs_ns,
         lhs_inv_ns=lhs_inv_ns,
+        y_eig=interior_eigen(D2),
     )
 
 

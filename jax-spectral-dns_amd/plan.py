@@ -49,6 +49,7 @@ class StepPlan:
         explicit_dt: Optional[float] = None,
         plan_re: Optional[float] = None,
         adjoint: bool = False,
+        eigen_solve: bool = True,
     ):
         if not torch.cuda.is_available():
             raise _lib.JsdError("no GPU visible: the channel-flow step runs only through the HIP library")
@@ -91,6 +92,9 @@ class StepPlan:
         desc.base_du = _ptr(base_du)
         desc.y_weights = _ptr(arr(g.y_weights))
         desc.adjoint = int(bool(adjoint))
+        # eigen form of the implicit solve (shared E, E^-1 per parity half
+        # instead of per-|k| inverse tables); False keeps the tables
+        desc.y_eig = _ptr(arr(o.y_eig)) if eigen_solve else None
         self.has_adjoint = bool(adjoint) or (kind == _lib.KIND_PERTURBATION and self.linearise)
         h = ctypes.c_void_p()
         _lib.check(lib.jsd_plan_create(ctypes.byref(desc), ctypes.byref(h)))

@@ -77,6 +77,58 @@ def test_operator_tables_match_oracle():
     assert np.abs(ops.lhs_inv_ns - ref.lhs_inv_ns).max() < 1e-12
 
 
+@pytest.mark.parametrize("ny", [33, 48, 129, 130])
+def test_interior_eigen_reproduces_inverse_tables(ny):
+    """The eigen form of the implicit solve: on each parity half of the
+    interior rows, E diag(1/((1 + c k^2) - c lam)) E^-1 is the interior block
+    of lhs_inv[s] (c = beta_s dt/Re) and E diag(1/(lam - k^2)) E^-1 that of
+    the Poisson inverse, for every |k| class tested; E E^-1 = I and
+    D^2 E = E diag(lam) to rounding.  A sign slip in c or k^2, a swapped
+    parity block or a dropped centre column fails here."""
+    shape = (8, ny, 8)
+    g = DM.make_grid(shape, 2 * np.pi, np.pi)
+    re, dt = 2500.0, 0.004
+    ops = DM.build_operators(g, re, dt)
+    n = g.ny
+    ne, no = n // 2 - 1 + n % 2, n // 2 - 1
+    f = ops.y_eig
+    sizes = [ne * ne, no * no, ne * ne, no * no, ne, no]
+    parts = np.split(f, np.cumsum(sizes)[:-1])
+    Ee, Eo = parts[0].reshape(ne, ne), parts[1].reshape(no, no)
+    Xe, Xo = parts[2].reshape(ne, ne), parts[3].reshape(no, no)
+    le, lo = parts[4], parts[5]
+    Be, Bo = DM.parity_blocks(g.D2)
+    for B, E, X, lam in ((Be, Ee, Xe, le), (Bo, Eo, Xo, lo)):
+        assert np.all(lam < 0)
+        assert np.abs(E @ X - np.eye(len(lam))).max() < 1e-13
+        assert np.abs(B @ E - E * lam).max() < 1e-12 * np.abs(B).max()
+    # parity basis of the interior: x_int = T_e e + T_o o
+    Te = np.zeros((n, ne))
+    To = np.zeros((n, no))
+    for h in range(ne):
+        Te[h + 1, h] = 1.0
+        Te[n - 2 - h, h] = 1.0
+    for h in range(no):
+        To[h + 1, h] = 1.0
+        To[n - 2 - h, h] = -1.0
+    Pe = np.linalg.pinv(Te)
+    Po = np.linalg.pinv(To)
+    for a, b in ((0, 0), (0, 1), (1, 0), (g.ka - 1, g.kb - 1), (2, 3)):
+        k2 = g.kx[a] ** 2 + g.kz[b] ** 2
+        for s in range(3):
+            c = DM.RK_BETA[s] * dt / re
+            Li = ops.lhs_inv[s, a, b]
+            P = ops.poisson[a, b]
+            for T, Pr, E, X, lam in ((Te, Pe, Ee, Xe, le), (To, Po, Eo, Xo, lo)):
+                dl = 1.0 / ((1.0 + c * k2) - c * lam)
+                helm = T @ (E * dl) @ X @ Pr
+                pois = T @ (E * (dl / (lam - k2))) @ X @ Pr
+                ref_h = Li @ T @ Pr
+                ref_p = P @ Li @ T @ Pr
+                assert np.linalg.norm(helm - ref_h) < 1e-11 * np.linalg.norm(ref_h)
+                assert np.linalg.norm(pois - ref_p) < 1e-10 * np.linalg.norm(ref_p)
+
+
 def test_wall_correction_solution_satisfies_neumann():
     """v_p + a v_a + b flip(v_a) has zero wall derivative for any v_p."""
     g = DM.make_grid((24, 33, 8))
