@@ -39,7 +39,7 @@ KERNEL_NAMES = {
     "fft_z_mid": "k_fft_z_mid_256<6, 0>",
     "fft_x_fwd": "k_fft_x_fwd_t<Rad<8, 5, 5, 1> >",
     "ygemm_hv": "k_ygemm<1>",
-    "solve": "k_solve",
+    "solve": "k_solve_eig",
     "nl_lin": "k_nl_lin",
 }
 
@@ -52,7 +52,8 @@ def algorithmic_bytes(g, nsub: int = 3) -> dict:
     eigenvector blocks E, E^-1 of the interior D^2 halves (2 (nE^2 + nO^2)
     doubles, once), the v_a parity tables (4 x 64 + 2 doubles per |k|) and
     12.33 S of state: u, v, w, H, on average 4/3 old-term fields; it writes
-    u, v, w, h_v, h_g."""
+    u, v, w, h_v, h_g, and in two of the three substeps D u, D v, D w of the
+    new state (the next substep's y-derivatives)."""
     S = 16.0 * g.nxm * g.ny * g.nzm
     T = 16.0 * g.ppx * g.ny * g.nzm
     nO = g.ny // 2 - 1
@@ -65,7 +66,7 @@ def algorithmic_bytes(g, nsub: int = 3) -> dict:
         "fft_z_mid": 8 * T,  # read 6 T, write 2 T
         "fft_x_fwd": 2 * T + 3 * S,  # read 2 T, write 3 components of H
         "ygemm_hv": 3 * S + 2 * S,  # read H, write h_v, h_g
-        "solve": tables + (3 + 3 + 2.0 * (nsub - 1) / nsub + 3 + 2) * S,
+        "solve": tables + (3 + 3 + 2.0 * (nsub - 1) / nsub + 3 + 2 + 3.0 * (nsub - 1) / nsub) * S,
         "nl_lin": 4 * S + 3 * S,
     }
 
@@ -102,15 +103,17 @@ def roofline_entry(dominant, mfma_bound, nflops, nbytes, dom_ms, achieved_gbs, t
 
 
 def solve_flops(g) -> float:
-    """FP64 MFMA flops of one solve launch on the interior even/odd blocks
-    (nE x nO for D, nE^2 + nO^2 for the centro-symmetric operators), per
-    pencil: D X and D v_p (2 real columns each), D^2 v (2), the rhs matrix,
-    E^-1 and E on [phi | omega_y] (4 each)."""
+    """FP64 MFMA flops of one solve launch in eigen coordinates, averaged over
+    the three substeps, on the interior even/odd blocks (nE x nO for the
+    parity-swapping E^-1 D and D E, nE^2 + nO^2 for E and E^-1), per pencil
+    and real column: E^-1 on v, omega_y, H1, h_g (8 columns), E^-1 D on X (2),
+    E on [z_v | z_w] (4), D E on z_v (2); in two of three substeps also
+    D E on z_w (2) and E on lam z_v (2) for the derivatives of the new state."""
     nO = g.ny // 2 - 1
     nE = nO + (g.ny & 1)
     d1 = 4.0 * nE * nO  # per real column
     d2 = 2.0 * (nE * nE + nO * nO)
-    return g.nxm * g.nzm * (4 * d1 + 14 * d2)
+    return g.nxm * g.nzm * (12 * d2 + 4 * d1 + (2.0 / 3.0) * (2 * d1 + 2 * d2))
 
 
 FLOPS = {"ygemm": ygemm_flops, "solve": solve_flops}

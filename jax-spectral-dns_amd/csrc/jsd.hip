@@ -1244,6 +1244,7 @@ struct jsd_plan {
   double *DEO, *DOE, *D2E, *D2O, *RBP, *LIP, *POP, *CW, *RW, *VAP, *VINV;  // parity blocks of the solve
   double *EVF, *EIF, *LAM;  // eigen-form solve: E, E^-1 fragments [even, odd], eigenvalues [2][PH]
   double *FF, *GF, *CWE, *RWE;  // E^-1 D and D E fragments, wall columns / rows in eigen coordinates
+  double *VD2, *Dm;             // D^2 v_a parity tables with wall parts; D row-major (mean-flow derivatives)
   double c_impl[3];         // beta_s dt / Re of the Helmholtz operators
   double *EH_R, *EG_R, *LAMR;  // adjoint eigen form: full-row E and E^-1 (row-major), eigenvalue per row
   double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
@@ -1418,7 +1419,7 @@ static const double RK_BET[3] = {37.0 / 160.0, 5.0 / 24.0, 1.0 / 6.0};
 // The factorisation is checked against the lhs_inv / poisson tables on a
 // sample of wavenumbers and every substep before it replaces them.
 static int eigen_tables(jsd_plan* p, const jsd_plan_desc* d, int N, size_t nk, const double* deo, const double* doe,
-                        const double* cw, const double* rw) {
+                        const double* cw, const double* rw, const double* rw2) {
   const int nE = p->nE, nO = p->nO;
   const double* f = d->y_eig;
   const double* Ee = f;
@@ -1525,7 +1526,7 @@ static int eigen_tables(jsd_plan* p, const jsd_plan_desc* d, int N, size_t nk, c
   // F = E^-1 D (even coordinates <- odd rows: X_e DEO; odd <- even: X_o DOE),
   // G = D E (even rows <- odd coordinates: DEO E_o; odd <- even: DOE E_e),
   // the wall columns CW and wall rows RW of D, D^2 carried to the coordinates
-  std::vector<double> ff(2 * (size_t)PFRAG), gf(2 * (size_t)PFRAG), cwe(4 * PH, 0.0), rwe(2 * PH, 0.0);
+  std::vector<double> ff(2 * (size_t)PFRAG), gf(2 * (size_t)PFRAG), cwe(4 * PH, 0.0), rwe(4 * PH, 0.0);
   auto prod = [&](const double* A, int lda, int ar, int ak, const double* B, int ldb, int bc, double* out) {
     std::fill(blk.begin(), blk.end(), 0.0);
     for (int i = 0; i < ar; ++i)
@@ -1561,6 +1562,17 @@ static int eigen_tables(jsd_plan* p, const jsd_plan_desc* d, int N, size_t nk, c
     long double acc = 0.0L;
     for (int h = 0; h < nE; ++h) acc += (long double)rw[PH + h] * Ee[(size_t)h * nE + c];
     rwe[PH + c] = (double)acc;
+  }
+  // D^2 wall rows: RWE[2] = RW2[0] E_e (even wall part), RWE[3] = RW2[1] E_o
+  for (int c = 0; c < nE; ++c) {
+    long double acc = 0.0L;
+    for (int h = 0; h < nE; ++h) acc += (long double)rw2[h] * Ee[(size_t)h * nE + c];
+    rwe[2 * PH + c] = (double)acc;
+  }
+  for (int c = 0; c < nO; ++c) {
+    long double acc = 0.0L;
+    for (int h = 0; h < nO; ++h) acc += (long double)rw2[PH + h] * Eo[(size_t)h * nO + c];
+    rwe[3 * PH + c] = (double)acc;
   }
   int rc = upload(p, &p->EVF, frag.data(), frag.size());
   if (!rc) rc = upload(p, &p->EIF, ifrag.data(), ifrag.size());
@@ -1788,7 +1800,13 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     std::vector<double> deo((size_t)PH * PH), doe((size_t)PH * PH);
     parity_dense(d->D, N, -1, 1, 0, deo.data());
     parity_dense(d->D, N, -1, 0, 1, doe.data());
-    if (d->y_eig) TRY(eigen_tables(p, d, N, nk, deo.data(), doe.data(), cw.data(), rw.data()));
+    // wall rows of D^2 over the interior parity halves: even wall part from
+    // even input, odd wall part from odd input (D^2 keeps parities)
+    std::vector<double> rw2(2 * PH, 0.0);
+    for (int h = 0; h < p->nE; ++h) rw2[h] = parity_entry(D2.data(), N, +1, 0, 0, h);
+    for (int h = 0; h < p->nO; ++h) rw2[PH + h] = parity_entry(D2.data(), N, +1, 1, 0, h);
+    if (d->y_eig)
+      TRY(eigen_tables(p, d, N, nk, deo.data(), doe.data(), cw.data(), rw.data(), rw2.data()));
     std::vector<double> vap(3 * nk * 4 * (size_t)PH, 0.0), vinv(3 * nk * 2);
     for (size_t k = 0; k < 3 * nk; ++k) {
       const double* va = d->v_a + k * N;
@@ -1816,6 +1834,34 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     }
     TRY(upload(p, &p->VAP, vap.data(), vap.size()));
     TRY(upload(p, &p->VINV, vinv.data(), vinv.size()));
+    p->VD2 = p->Dm = nullptr;
+    if (p->EVF) {
+      // D^2 v_a[s][k] on the interior parity halves and its wall parts (the
+      // eigen-form solve also returns the y-derivatives of the new state)
+      std::vector<double> d2e((size_t)PH * PH), d2o((size_t)PH * PH), vd2(3 * nk * (2 * (size_t)PH + 2), 0.0);
+      parity_dense(D2.data(), N, +1, 0, 0, d2e.data());
+      parity_dense(D2.data(), N, +1, 1, 1, d2o.data());
+      for (size_t k = 0; k < 3 * nk; ++k) {
+        const double* o = vap.data() + k * 4 * PH;
+        double* r = vd2.data() + k * (2 * PH + 2);
+        long double we = 0.0L, wo = 0.0L;
+        for (int h = 0; h < PH; ++h) {
+          long double a1 = 0.0L, a2 = 0.0L;
+          for (int g2 = 0; g2 < PH; ++g2) {
+            a1 += (long double)d2e[(size_t)h * PH + g2] * o[g2];
+            a2 += (long double)d2o[(size_t)h * PH + g2] * o[PH + g2];
+          }
+          r[h] = (double)a1;
+          r[PH + h] = (double)a2;
+          we += (long double)rw2[h] * o[h];
+          wo += (long double)rw2[PH + h] * o[PH + h];
+        }
+        r[2 * PH] = (double)we;
+        r[2 * PH + 1] = (double)wo;
+      }
+      TRY(upload(p, &p->VD2, vd2.data(), vd2.size()));
+      TRY(upload(p, &p->Dm, d->D, nn));
+    }
   }
   p->DFT1 = p->D_R = p->D2_R = p->RB_R = p->LI_R = p->PO_R = nullptr;
   p->EH_R = p->EG_R = p->LAMR = nullptr;
@@ -1978,13 +2024,13 @@ static NlLinArgs nl_lin_args(const jsd_plan* p, const double2* U) {
 }
 
 // H (3 comps, stored convention) from the state U (and the forward state for a nonlinear dual)
-static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStream_t st) {
+static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStream_t st, bool have_du) {
   const jsd_plan_desc& d = p->d;
   const size_t fs = p->fs;
   const int M = p->M, N = p->N, Z = p->Z;
   const bool lin = (d.kind != JSD_KIND_NS) && d.linearise;
   if (lin) {
-    int rc = launch_ygemm_plain(p, U + fs, p->DU, 1, 1, st);
+    int rc = have_du ? JSD_OK : launch_ygemm_plain(p, U + fs, p->DU, 1, 1, st);
     if (rc) return rc;
     const size_t tot = fs;
     prof_begin(p, 6, st);
@@ -1993,7 +2039,7 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     HIPCHK(hipGetLastError());
     return JSD_OK;
   }
-  int rc = launch_ygemm_plain(p, U, p->DU, 3, 1, st);
+  int rc = have_du ? JSD_OK : launch_ygemm_plain(p, U, p->DU, 3, 1, st);
   if (rc) return rc;
   const int nqb = (p->KZ + 1 + QB - 1) / QB;
   const int ZT = p->ZT;
@@ -2137,6 +2183,10 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
   S.CWE = p->CWE;
   S.RWE = p->RWE;
   S.LAM = p->LAM;
+  S.VD2 = p->VD2 ? p->VD2 + (size_t)s * nk * (2 * PH + 2) : nullptr;
+  S.Dm = p->Dm;
+  S.DUo = nullptr;
+  S.du_full = 0;
   S.c_impl = p->EVF ? p->c_impl[s] : 0.0;
   S.CW = p->CW;
   S.RW = p->RW;
@@ -2164,12 +2214,27 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
   return S;
 }
 
-static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st) {
-  int rc = nonlinear_H(p, U, UF, st);
+// chain = true: p->DU already holds D U (the previous substep's solve wrote
+// it) and this substep's solve writes D of its new state for the next one.
+static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st,
+                   bool have_du = false, bool write_du = false) {
+  int rc = nonlinear_H(p, U, UF, st, have_du);
   if (rc) return rc;
   const int nblk = ((solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
+  SolveArgs S = solve_args(p, U, s, dPdx, roles_of(p));
+  if (write_du) {
+    S.DUo = p->DU;
+    S.du_full = (p->d.kind == JSD_KIND_NS) || !p->d.linearise;  // linearised steps need D v only
+  }
   prof_begin(p, 5, st);
-  hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, solve_args(p, U, s, dPdx, roles_of(p)));
+  if (!p->EVF)
+    hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, S);
+  else if (!S.DUo)
+    hipLaunchKernelGGL(k_solve_eig<0>, dim3(nblk), dim3(256), 0, st, S);
+  else if (S.du_full)
+    hipLaunchKernelGGL(k_solve_eig<2>, dim3(nblk), dim3(256), 0, st, S);
+  else
+    hipLaunchKernelGGL(k_solve_eig<1>, dim3(nblk), dim3(256), 0, st, S);
   prof_end(p, 5, st);
   HIPCHK(hipGetLastError());
   std::swap(p->HV, p->HVo);
@@ -2183,9 +2248,15 @@ extern "C" int jsd_step(jsd_plan* p, void* vel_hat, double dPdx, int nsteps, voi
   if (p->d.kind == JSD_KIND_DUAL && !p->d.linearise)
     JSD_FAIL(JSD_ERR_STATE, "a nonlinear dual plan steps with jsd_dual_step");
   hipStream_t st = (hipStream_t)stream;
+  // nonlinear steps in eigen form: the solves of substeps 1 and 2 also return
+  // D u_c of their new state, so substeps 2 and 3 skip the y-derivative
+  // kernel.  The chain stays inside a step, so a trajectory does not depend
+  // on how its steps are grouped into calls (checkpointed recomputation
+  // reproduces it bit for bit).
+  const bool chain = p->EVF != nullptr;  // NS, perturbation and linearised dual plans
   for (int n = 0; n < nsteps; ++n)
     for (int s = 0; s < 3; ++s) {
-      int rc = substep(p, (double2*)vel_hat, nullptr, s, dPdx, st);
+      int rc = substep(p, (double2*)vel_hat, nullptr, s, dPdx, st, chain && s > 0, chain && s < 2);
       if (rc) return rc;
     }
   return JSD_OK;
@@ -2211,7 +2282,7 @@ extern "C" int jsd_nonlinear(jsd_plan* p, const void* vel_hat, const void* fwd_h
                              void* conv_ns, void* stream) {
   if (!p || !vel_hat || !h_v || !h_g || !conv_ns) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   hipStream_t st = (hipStream_t)stream;
-  int rc = nonlinear_H(p, (const double2*)vel_hat, (const double2*)fwd_hat, st);
+  int rc = nonlinear_H(p, (const double2*)vel_hat, (const double2*)fwd_hat, st, false);
   if (rc) return rc;
   rc = launch_hv(p, p->H, (double2*)h_v, (double2*)h_g, p->CONV, st);
   if (rc) return rc;
@@ -2544,10 +2615,12 @@ struct OpNlLinT {
 struct OpSolve {
   typedef SolveArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) {
-    if (a.EVF)
-      solve_body_eig(a, k.x);
-    else
+    if (!a.EVF)
       solve_body(a, k.x);
+    else if (!a.DUo)
+      solve_body_eig<0>(a, k.x);
+    else
+      solve_body_eig<1>(a, k.x);  // batches hold linearised runs: D v only
   }
 };
 struct OpSolveT {
@@ -2585,6 +2658,7 @@ struct BatchStage {
 struct jsd_batch {
   int nrun = 0;
   bool vjp = false;
+  bool chain = false;  // every plan in eigen form: substeps 2, 3 take D v from the solve
   std::vector<jsd_plan*> plans;
   std::vector<double2*> state, adj;
   std::vector<void*> allocs;
@@ -2633,7 +2707,8 @@ static int launch_stage(const jsd_batch* b, const BatchStage& S, int cap, hipStr
 
 static int batch_fwd_step(const jsd_batch* b, hipStream_t st) {
   for (int s = 0; s < 3; ++s) {
-    int rc = launch_stage<OpYpar>(b, b->yg, b->cap[BS_YG], st);
+    // the solves of substeps 1 and 2 return D v of their new state (eigen form)
+    int rc = (s > 0 && b->chain) ? JSD_OK : launch_stage<OpYpar>(b, b->yg, b->cap[BS_YG], st);
     if (!rc) rc = launch_stage<OpNlLin>(b, b->nl, b->cap[BS_NL], st);
     if (!rc) rc = launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
     if (rc) return rc;
@@ -2746,6 +2821,8 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
     b->state.push_back((double2*)states[r]);
     b->adj.push_back(b->vjp ? (double2*)adjoints[r] : nullptr);
   }
+  b->chain = true;
+  for (int r = 0; r < nrun; ++r) b->chain = b->chain && plans[r]->EVF != nullptr;
 #define BTRY(x)               \
   do {                        \
     int _rc = (x);            \
@@ -2780,7 +2857,12 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       for (int r = 0; r < nrun; ++r) {
         jsd_plan* p = b->plans[r];
         const Roles r0 = roles_of(p);
-        sa.push_back(solve_args(p, b->state[r], s, 0.0, (s & 1) ? roles_swapped(r0) : r0));
+        SolveArgs S = solve_args(p, b->state[r], s, 0.0, (s & 1) ? roles_swapped(r0) : r0);
+        if (b->chain && s < 2) {
+          S.DUo = p->DU;
+          S.du_full = 0;
+        }
+        sa.push_back(S);
       }
       BTRY(make_stage(b, b->solve[s], sa, sg));
     }

@@ -215,3 +215,31 @@ def test_eigen_solve_matches_table_solve(kind, shape):
             pe.step(a, -2.0 / 5000.0 if kind == KIND_NS else 0.0, 1)
             pt.step(b, -2.0 / 5000.0 if kind == KIND_NS else 0.0, 1)
         assert rel(a, b) < 1e-12 * (n + 1), n
+
+
+@pytest.mark.parametrize("kind", [KIND_NS, KIND_PERTURBATION])
+def test_multistep_call_chains_derivatives(kind):
+    """In eigen form the y-derivatives of the state in substeps 2 and 3 come
+    from the previous substep's solve instead of the y-derivative kernel.
+    Three steps in one call match the oracle and are bit-identical to three
+    single-step calls (the chain never crosses a step boundary)."""
+    g, d = pair((24, 65, 16), 2 * np.pi, np.pi)
+    if kind == KIND_NS:
+        re, dt, dpdx = 5000.0, 0.005, -2.0 / 5000.0
+        u = ns_field(g, seed=31)
+        sp = O.make_ns(d, re, dt, dpdx)
+    else:
+        re, dt, dpdx = 3000.0, 0.01, 0.0
+        u = perturbation_field(g, seed=32, energy=1e-2)
+        sp = O.make_perturbation(d, re, dt, False)
+    plan = StepPlan(g, re, dt, kind)
+    ref = u.clone()
+    for n in range(3):
+        ref = O.rk_step(sp, ref, n)
+    a = u.to(DEV).clone()
+    plan.step(a, dpdx, 3)
+    b = u.to(DEV).clone()
+    for _ in range(3):
+        plan.step(b, dpdx, 1)
+    assert rel(a, ref) < 3e-11
+    assert torch.equal(a, b)
