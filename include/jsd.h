@@ -136,12 +136,13 @@ int jsd_energy_grad(jsd_plan* plan, const void* vel_hat, void* out, double scale
 int jsd_ydiff(jsd_plan* plan, const void* in, void* out, int ncomp, int order, void* stream);
 
 /* Instrumentation, no counterpart in the reference (its only timer is the
- * wall clock around solve_scan, equation.py:347-401): with enable != 0 every
- * later kernel launch is bracketed by HIP events on its stream; a call with
- * ms / count != NULL first returns the accumulated milliseconds and launch
- * counts of the 7 kernel classes (y-derivative GEMM, inverse x FFT, z
- * products pass, forward x FFT, h_v GEMM, implicit solve, linearised term),
- * then resets them. */
+ * wall clock around solve_scan, equation.py:347-401).  enable is a bit mask
+ * over the 7 kernel classes (bit 0 y-derivative GEMM, 1 inverse x FFT, 2 z
+ * products pass, 3 forward x FFT, 4 h_v GEMM, 5 implicit solve, 6 linearised
+ * term; 0x7f all, 0 none): every later launch of a selected class is
+ * bracketed by HIP events on its stream.  A call with ms / count != NULL
+ * first returns the accumulated milliseconds and launch counts of the
+ * classes, then resets them. */
 int jsd_profile(jsd_plan* plan, int enable, double* ms, long* count);
 
 /* ---- batches of independent runs ------------------------------------------

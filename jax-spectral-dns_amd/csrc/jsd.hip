@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -439,13 +440,14 @@ enum { PROD_NS = 0, PROD_PERT = 1, PROD_DUAL = 2 };
 // Skew-symmetric products at one physical point.  row[2c] = (u_c, d_x u_c),
 // row[2c+1] = (d_y u_c, d_z u_c); the dual adds the forward field in rows
 // 6..11 (the base flow U(y) is added here).  H gets the three components.
-template <int PROD>
+// row[t * ST] holds packed transform t at one physical point.
+template <int PROD, int ST = 1>
 __device__ __forceinline__ void zmid_products(const double2* row, double U, double dU, double clin, double* H) {
   double u[3], g[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double2 r0 = row[2 * c];
-    const double2 r1 = row[2 * c + 1];
+    const double2 r0 = row[(2 * c) * ST];
+    const double2 r1 = row[(2 * c + 1) * ST];
     u[c] = r0.x;
     g[c][0] = r0.y;
     g[c][1] = r1.x;
@@ -478,8 +480,8 @@ __device__ __forceinline__ void zmid_products(const double2* row, double U, doub
     double wv[3], gw[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double2 r0 = row[6 + 2 * c];
-      const double2 r1 = row[7 + 2 * c];
+      const double2 r0 = row[(6 + 2 * c) * ST];
+      const double2 r1 = row[(7 + 2 * c) * ST];
       wv[c] = clin * r0.x;
       gw[c][0] = clin * r0.y;
       gw[c][1] = clin * r1.x;
@@ -791,7 +793,9 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
 template <int NT>
 __host__ __device__ constexpr int z256_threads() { return NT * 16 > 128 ? NT * 16 : 128; }
 
-template <int NT, int PROD>
+// PZ: at most 129 stored kz (|kz| <= 64), so step 1 of the inverse sees
+// zeros at n2 = 5..11 (dft16_pz).
+template <int NT, int PROD, bool PZ>
 __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
     const double2* __restrict__ T, long tstride, double2* __restrict__ V, long vstride,
     const double2* __restrict__ tw, int N, int Z, int ZT, double clin, const double* __restrict__ Ub,
@@ -799,7 +803,7 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
   constexpr int P = 256;
   constexpr int NTH = z256_threads<NT>();
   constexpr int ITEMS = (P + NTH - 1) / NTH;
-  // [t][n1][k2] (padded) for the exchanges, [z][t] for the products
+  // [t][n1][k2] (padded) for the exchanges, [t][z] for the products
   __shared__ double2 lds[NT * 16 * Z16_LD > P * NT ? NT * 16 * Z16_LD : P * NT];
   const int x = blockIdx.x;
   const int l = blockIdx.y;
@@ -818,7 +822,8 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
       else if (n >= P - nneg) j = n - P + Z;
       v[n2] = j >= 0 ? T[t * tstride + line + j] : make_double2(0.0, 0.0);
     }
-    dft16<+1>(v);
+    if constexpr (PZ) dft16_pz<+1>(v);
+    else dft16<+1>(v);
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
       double2 y = v[k2];
@@ -838,7 +843,7 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
   if (threadIdx.x < NT * 16) {
     const int t = threadIdx.x >> 4, k2 = threadIdx.x & 15;
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) lds[(16 * k1 + k2) * NT + t] = v[k1];
+    for (int k1 = 0; k1 < 16; ++k1) lds[t * P + 16 * k1 + k2] = v[k1];
   }
   __syncthreads();
   // ---- products at each physical point
@@ -848,7 +853,7 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q) {
     const int z = threadIdx.x + q * NTH;
-    if (z < P) zmid_products<PROD>(lds + z * NT, U, dU, clin, Hs[q]);
+    if (z < P) zmid_products<PROD, P>(lds + z, U, dU, clin, Hs[q]);
   }
   __syncthreads();
   // packed rows (H0 + i H1) and (H2 + 0 i) at [o][z]
@@ -1259,13 +1264,13 @@ struct jsd_plan {
   double* eout;
   int npartial;
   // optional per-kernel-class timing with HIP events on the launch stream
-  int prof_on;
+  int prof_on;  // bit mask of the bracketed kernel classes
   std::vector<hipEvent_t> prof_ev[JSD_NPROF];
   size_t prof_used[JSD_NPROF];
 };
 
 static void prof_begin(jsd_plan* p, int cls, hipStream_t st) {
-  if (!p->prof_on) return;
+  if (!(p->prof_on >> cls & 1)) return;
   auto& v = p->prof_ev[cls];
   const size_t k = p->prof_used[cls];
   if (v.size() < k + 2) {
@@ -1277,7 +1282,7 @@ static void prof_begin(jsd_plan* p, int cls, hipStream_t st) {
   (void)hipEventRecord(v[k], st);
 }
 static void prof_end(jsd_plan* p, int cls, hipStream_t st) {
-  if (!p->prof_on) return;
+  if (!(p->prof_on >> cls & 1)) return;
   auto& v = p->prof_ev[cls];
   const size_t k = p->prof_used[cls];
   if (v.size() < k + 2) return;
@@ -2095,15 +2100,22 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   const bool z_256 = ct && plan_is<RZ256>(p->fpz);
   if (z_256) {
     const dim3 grid(d.ppx, N);
-    if (prod == PROD_NS)
-      hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_NS>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts, p->V,
-                         (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
-    else if (prod == PROD_PERT)
-      hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_PERT>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts, p->V,
-                         (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+    auto launch = [&](auto pz) {
+      constexpr bool PZ = decltype(pz)::value;
+      if (prod == PROD_NS)
+        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_NS, PZ>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts,
+                           p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+      else if (prod == PROD_PERT)
+        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_PERT, PZ>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts,
+                           p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+      else
+        hipLaunchKernelGGL((k_fft_z_mid_256<12, PROD_DUAL, PZ>), grid, dim3(z256_threads<12>()), 0, st, p->T,
+                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+    };
+    if (Z <= 129)
+      launch(std::true_type());
     else
-      hipLaunchKernelGGL((k_fft_z_mid_256<12, PROD_DUAL>), grid, dim3(z256_threads<12>()), 0, st, p->T, (long)ts,
-                         p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+      launch(std::false_type());
   }
   const bool z_ct = z_256 || (ct && visit_z(p->fpz, [&](auto c) {
     using C = decltype(c);
@@ -2329,7 +2341,7 @@ extern "C" int jsd_profile(jsd_plan* p, int enable, double* ms, long* count) {
     }
   }
   for (int c = 0; c < JSD_NPROF; ++c) p->prof_used[c] = 0;
-  p->prof_on = enable ? 1 : 0;
+  p->prof_on = enable & ((1 << JSD_NPROF) - 1);
   return JSD_OK;
 }
 
@@ -2404,7 +2416,7 @@ static YgemmArgs acc_args(const jsd_plan* p, double2* UB) {
 
 static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
   const Roles r = roles_of(p);
-  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  const int nblk = ((solve_groups(p->EH_R != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
   hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));
@@ -2514,7 +2526,7 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStre
 
 static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s, bool has_carry, hipStream_t st) {
   const Roles r = roles_of(p);
-  const int nblk = ((p->Ka * p->Kb + 7) / 8) * 8;
+  const int nblk = ((solve_groups(p->EH_R != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
   hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));

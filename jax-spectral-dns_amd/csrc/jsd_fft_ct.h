@@ -130,6 +130,43 @@ __device__ __forceinline__ void dft16(double2* v) {
   }
 }
 
+// dft16 for inputs with v[5..11] == 0 (v[4] may be nonzero): the spectrum of
+// at most 129 stored modes (|k| <= 64) zero-padded to 256 and split as
+// n = n1 + 16 n2.  The first-stage 4-point DFTs drop the additions of the
+// known zeros (x + 0 and 0 - x are exact, so the result is bit-identical).
+template <int SIGN>
+__device__ __forceinline__ void dft16_pz(double2* v) {
+  constexpr double sign = (double)SIGN;
+  double2 a[4][4];  // a[n1][k2]
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    const double2 t0 = v[n1], t3 = v[n1 + 12];
+    double2 b0, b1;
+    if (n1 == 0) {
+      b0 = cadd(v[4], t3);
+      b1 = csub(v[4], t3);
+    } else {
+      b0 = t3;
+      b1 = make_double2(-t3.x, -t3.y);
+    }
+    const double2 b1r = make_double2(-sign * b1.y, sign * b1.x);
+    double2 t[4] = {cadd(t0, b0), cadd(t0, b1r), csub(t0, b0), csub(t0, b1r)};
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      const int m = n1 * k2;
+      if (m == 0) a[n1][k2] = t[k2];
+      else a[n1][k2] = cmul(t[k2], make_double2(W16::c[m], sign * W16::s[m]));
+    }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    double2 t[4] = {a[0][k2], a[1][k2], a[2][k2], a[3][k2]};
+    dft4(t, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[4 * k1 + k2] = t[k1];
+  }
+}
+
 // Full in-place FFT of NROWS rows already in DIT order; SIGN -1 forward, +1 inverse.
 template <class C, int NROWS, int PS, int RS, int NTH, int SIGN>
 __device__ __forceinline__ void fft_lds_ct(double2* buf, const double2* __restrict__ tw) {

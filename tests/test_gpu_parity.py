@@ -68,9 +68,10 @@ def test_nonlinear_ns(shape, lx, lz):
     assert rel(conv, rc) < 1e-12
 
 
-@pytest.mark.parametrize("shape", [(16, 33, 160)])
+@pytest.mark.parametrize("shape", [(16, 33, 160), (16, 33, 96), (8, 33, 128)])
 def test_nonlinear_ns_z256(shape):
-    """z transform length 256 (the 16 x 16 register kernel) against the oracle."""
+    """z transform length 256 (the 16 x 16 register kernel) against the oracle;
+    97 and 129 stored kz take its pruned first stage (inputs n2 = 5..11 zero)."""
     g, d = pair(shape)
     assert g.ppz == 256
     plan = StepPlan(g, 5000.0, 0.005, KIND_NS)
@@ -97,7 +98,7 @@ def test_nonlinear_ns_runtime_lengths(shape):
     assert rel(conv, rc) < 1e-12
 
 
-@pytest.mark.parametrize("shape", [(24, 33, 16), (16, 33, 160)])
+@pytest.mark.parametrize("shape", [(24, 33, 16), (16, 33, 160), (16, 33, 128)])
 @pytest.mark.parametrize("kind", [KIND_NS, KIND_DUAL])
 def test_compile_time_and_runtime_fft_agree(monkeypatch, kind, shape):
     """The compile-time-length FFT kernels (z length 256: the 16 x 16
