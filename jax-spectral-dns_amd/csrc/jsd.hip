@@ -338,6 +338,18 @@ __global__ __launch_bounds__(256) void k_ypar(YparArgs G) {
 // A WG of the x-passes owns QB kz "pairs" q = q0..q0+QB-1 (q <= KZ); slot c
 // (0..2QB-1) holds column j = q (c < QB) or j = Z - q (c >= QB, q >= 1).
 // The partner column -kz of slot c is slot c ^ QB (slot c itself for q = 0).
+// Streaming accesses (nontemporal: no reuse expected from the caches) for the
+// z pass's intermediates: each T row is read once and each V row written once.
+typedef double jsd_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+  jsd_d2v t = {v.x, v.y};
+  __builtin_nontemporal_store(t, (jsd_d2v*)p);
+}
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+  jsd_d2v t = __builtin_nontemporal_load((const jsd_d2v*)p);
+  return make_double2(t.x, t.y);
+}
+
 #define QB 4
 #define NSLOT (2 * QB)
 
@@ -820,7 +832,7 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
       int j = -1;
       if (n <= KZ) j = n;
       else if (n >= P - nneg) j = n - P + Z;
-      v[n2] = j >= 0 ? T[t * tstride + line + j] : make_double2(0.0, 0.0);
+      v[n2] = j >= 0 ? ld_nt(&T[t * tstride + line + j]) : make_double2(0.0, 0.0);
     }
     if constexpr (PZ) dft16_pz<+1>(v);
     else dft16<+1>(v);
@@ -896,7 +908,7 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
       int j = -1;
       if (b <= KZ) j = b;
       else if (b >= P - nneg) j = b - P + Z;
-      if (j >= 0) V[o * vstride + line + j] = v[k1];
+      if (j >= 0) st_nt(&V[o * vstride + line + j], v[k1]);
     }
   }
 }
