@@ -165,6 +165,45 @@ class NavierStokesVelVortPerturbationDual:
         return lam * u0 - v0, True
 
 
+    def get_projected_cg_grad(
+        self,
+        step_size: float,
+        beta: complex,
+        old_grad: torch.Tensor,
+        step_size_factor: Optional[float] = None,
+    ) -> Tuple[torch.Tensor, bool]:
+        """Conjugate-gradient direction under the energy constraint
+        (navier_stokes_perturbation_dual.py:1169-1201): the projected gradient
+        with the adjoint field v_0 replaced by v_0 - beta * old_grad."""
+        if self.v_0 is None:
+            self.run_backward_calculation()
+        f = 0.9 if step_size_factor is None else step_size_factor
+        p = self.forward_equation.plan()
+        u0 = self.velocity_u_hat_0
+        v0 = self.v_0 - beta * old_grad
+        e0 = p.energy(u0)
+        e0_adj = p.energy(v0)
+        c0 = 2 * 0.5 * (p.energy(u0 + v0) - e0 - e0_adj)  # 2 <u0, v0>_E
+        max_step = f * (e0_adj / e0 - c0**2 / (4 * e0**2)) ** (-1 / 2)
+        step_size = min(step_size, max_step)
+        A = (step_size * c0 - 2 * e0) / (2 * step_size * e0)
+        lam = A + math.sqrt(A**2 - (step_size * e0_adj - c0) / (step_size * e0))
+        return lam * u0 + (-1.0 * self.v_0 + beta * old_grad), True
+
+    def get_max_step_size(self, beta: complex = 0.0, old_grad: Optional[torch.Tensor] = None) -> float:
+        """Largest step the energy constraint admits
+        (gradient_descent_solver.py:510-520, ConjugateGradientDescentSolver.get_max_step_size_ls)."""
+        if self.v_0 is None:
+            self.run_backward_calculation()
+        p = self.forward_equation.plan()
+        u0 = self.velocity_u_hat_0
+        v0 = self.v_0 if old_grad is None else self.v_0 - beta * old_grad
+        e0 = p.energy(u0)
+        e0_adj = p.energy(v0)
+        c0 = 2 * 0.5 * (p.energy(u0 + v0) - e0 - e0_adj)
+        return (e0_adj / e0 - c0**2 / (4 * e0**2)) ** (-1 / 2)
+
+
 def perform_step_navier_stokes_perturbation_dual(nse: NavierStokesVelVortPerturbation, step_size: float = 1e-2):
     """navier_stokes_perturbation_dual.py:1204-1212."""
     dual = NavierStokesVelVortPerturbationDual.FromNavierStokesVelVortPerturbation(nse)

@@ -1,0 +1,66 @@
+"""Host-side linear stability / optimal transient growth
+(linear_stability_calculation.py:34-638), pinned on published answers:
+Orszag's (1971) least stable Orr-Sommerfeld eigenvalue at Re = 10000,
+alpha = 1, and the Reddy & Henningson (1993) transient-growth curve at
+Re = 3000, alpha = 1, beta = 0 that the reference ships
+(rh93_transient_growth.csv, digitised; copied as a fixture)."""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from jax_spectral_dns_amd.domain import PhysicalDomain
+from jax_spectral_dns_amd.linear_stability_calculation import LinearStabilityCalculation, clenshaw_curtis_weights
+from oracle import jsd_oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.parametrize("n", [9, 10, 33, 64])
+def test_clenshaw_curtis_is_exact_on_polynomials(n):
+    y = np.cos(np.pi * np.arange(n) / (n - 1))
+    w = clenshaw_curtis_weights(n)
+    for p in range(0, n - 1):
+        exact = 0.0 if p % 2 else 2.0 / (p + 1)
+        assert abs((w * y**p).sum() - exact) < 1e-13
+
+
+def test_orszag_eigenvalue():
+    """Least stable mode of plane Poiseuille flow at Re = 10000, alpha = 1:
+    c = 0.23752649 + 0.00373967 i (Orszag, J. Fluid Mech. 50, 1971)."""
+    lsc = LinearStabilityCalculation(Re=10000.0, alpha=1.0, beta=0.0, n=120)
+    lam, _ = lsc.calculate_eigenvalues()
+    c = 1j * lam[0] / lsc.alpha
+    assert abs(c.real - 0.23752649) < 1e-8
+    assert abs(c.imag - 0.00373967) < 1e-8
+
+
+def test_transient_growth_matches_reddy_henningson_1993():
+    """G(t) at Re = 3000, alpha = 1, beta = 0 against the digitised curve: within
+    3 % on t in [2, 44] (digitisation), peak 20.36 near t = 15."""
+    data = np.genfromtxt(os.path.join(HERE, "golden", "rh93_transient_growth.csv"), delimiter=",").T
+    lsc = LinearStabilityCalculation(Re=3000.0, alpha=1.0, beta=0.0, n=100)
+    lsc.calculate_transient_growth_max_energy(15.0, 60, Ts=list(data[0]))
+    G = np.array(lsc.Gs)
+    assert abs(G[0] - 1.0) < 1e-12
+    sel = (data[0] > 2.0) & (data[0] < 44.0)
+    assert np.all(np.abs(G[sel] - data[1][sel]) / data[1][sel] < 0.03)
+    assert abs(max(G) - 20.36) < 0.05
+
+
+def test_optimal_initial_condition_is_divergence_free_and_scaled():
+    """The optimal initial condition on a DNS grid (n = Ny): a single (alpha, beta)
+    wave, divergence free, at the requested energy."""
+    dom = PhysicalDomain.create((8, 33, 8), (True, False, True), scale_factors=(2 * np.pi, 1.0, 2 * np.pi))
+    lsc = LinearStabilityCalculation(Re=1000.0, alpha=1.0, beta=1.0, n=33)
+    u = lsc.calculate_transient_growth_initial_condition(dom, 10.0, 30, energy=1e-4)
+    d = O.make_domain((8, 33, 8))
+    div = O.diff(d, u[0], 0, 1) + O.diff(d, u[1], 1, 1) + O.diff(d, u[2], 2, 1)
+    assert torch.linalg.norm(div) < 1e-9 * torch.linalg.norm(u)
+    assert abs(float(O.energy(d, u)) - 1e-4) < 1e-12
+    mags = torch.linalg.norm(u, dim=(0, 2))  # per (kx, kz)
+    nz = [tuple(ij) for ij in torch.nonzero(mags > 1e-12 * mags.max()).tolist()]
+    M, Z = mags.shape
+    assert sorted(nz) == [(1, 1), (M - 1, Z - 1)]  # (alpha, beta) and its conjugate only

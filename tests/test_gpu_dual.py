@@ -87,3 +87,44 @@ def test_steepest_adaptive_descent_increases_gain_monotonically():
     assert vals[-1] > vals[0]
     e = solver.dual.forward_equation.plan().energy(solver.current_guess)
     assert abs(e - solver.e_0) <= 1e-12 * solver.e_0
+
+
+def test_projected_cg_grad_reduces_to_projected_grad():
+    """get_projected_cg_grad (navier_stokes_perturbation_dual.py:1169-1201) with
+    beta = 0 is the projected gradient (:1122-1167), and with beta != 0 it is
+    the projected gradient of the shifted adjoint field v_0 - beta g_old: the
+    returned direction d satisfies the energy constraint
+    E(u_0 + s d) = E(u_0) when s is below the admissible step."""
+    shape, lx, lz, re, dt, n = (24, 65, 4), 2 * np.pi, 2 * np.pi, 3000.0, 0.01, 50
+    g, _ = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=2, energy=1e-6)
+    dual = NavierStokesVelVortPerturbationDual(_nse(shape, lx, lz, re, dt, n * dt, True, u0))
+    s = 1e-3
+    g0, _ = dual.get_projected_grad(s)
+    g1, _ = dual.get_projected_cg_grad(s, 0.0, torch.zeros_like(g0))
+    assert torch.equal(g0, g1)
+    p = dual.forward_equation.plan()
+    u = dual.velocity_u_hat_0
+    e0 = p.energy(u)
+    gb, _ = dual.get_projected_cg_grad(s, 0.3, g0)
+    assert abs(p.energy(u + s * gb) - e0) < 1e-10 * e0
+    assert s < 0.9 * dual.get_max_step_size(0.3, g0)
+
+
+def test_conjugate_gradient_descent_increases_gain():
+    """ConjugateGradientDescentSolver (gradient_descent_solver.py:490-860,
+    default path without line search): a few iterations raise the linear
+    transient-growth gain at fixed energy, beta stays >= 0."""
+    from jax_spectral_dns_amd.gradient_descent_solver import ConjugateGradientDescentSolver
+
+    shape, lx, lz, re, dt, n = (24, 65, 4), 2 * np.pi, 2 * np.pi, 3000.0, 0.01, 100
+    g, _ = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=0, energy=1e-6)
+    solver = ConjugateGradientDescentSolver(_nse(shape, lx, lz, re, dt, n * dt, True, u0), step_size=1e-2, max_iterations=5)
+    vals = solver.optimise()
+    print("gains", vals, "betas", solver.betas)
+    assert len(vals) >= 3
+    assert max(vals[1:]) > vals[0]
+    assert all(b >= 0.0 for b in solver.betas)
+    e = solver.plan.energy(solver.current_guess)
+    assert abs(e - solver.e_0) <= 1e-12 * solver.e_0
