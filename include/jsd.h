@@ -74,6 +74,10 @@ typedef struct jsd_plan_desc {
                                inverses are then applied as E diag(.) E^-1 with
                                beta_s |dt/Re| from dt and re_tau; checked against
                                lhs_inv / poisson at plan creation */
+  int no_coupling_term;     /* perturbation plans: nonzero adds d_z hat(U' v) to h_g, the
+                               reference's coupling_term = False
+                               (navier_stokes_perturbation.py:214-224), as the time-dependent
+                               base flow uses (navier_stokes_perturbation_instationary.py:95-108) */
 } jsd_plan_desc;
 
 /* Replaces NavierStokesVelVort.__init__ / assemble_matrices
@@ -126,6 +130,13 @@ int jsd_step_vjp(jsd_plan* plan, void* grad, int nsteps, void* stream);
  * 9 spectral fields of recompute storage, kept by the plan.  Linearised
  * perturbation plans ignore state and run jsd_step_vjp. */
 int jsd_step_vjp_state(jsd_plan* plan, const void* state, void* grad, double dPdx, void* stream);
+
+/* Replaces the base flow of a perturbation plan (ny values of U(y) and U'(y),
+ * host arrays, copied before return): the time-dependent base flow of
+ * NavierStokesVelVortPerturbationInstationary (vel_base_fn,
+ * navier_stokes_perturbation_instationary.py:39-80) is set before each step.
+ * Not for plans in a batch while it steps. */
+int jsd_set_base_flow(jsd_plan* plan, const double* base_u, const double* base_du, void* stream);
 
 /* out = scale * dE/d(vel_hat) on the real-pair view, E the energy of
  * jsd_energy (field.py:607-611). */

@@ -49,6 +49,7 @@ class PlanDesc(ctypes.Structure):
         ("y_weights", _dp),
         ("adjoint", ctypes.c_int),
         ("y_eig", _dp),
+        ("no_coupling_term", ctypes.c_int),
     ]
 
 
@@ -71,6 +72,7 @@ EXPORTS = (
     "jsd_step_vjp_state",
     "jsd_energy_grad",
     "jsd_profile",
+    "jsd_set_base_flow",
     "jsd_batch_create",
     "jsd_batch_destroy",
     "jsd_batch_step",
@@ -100,6 +102,7 @@ def load() -> ctypes.CDLL:
     lib.jsd_step_vjp_state.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
+    lib.jsd_set_base_flow.argtypes = [vp, _dp, _dp, vp]
     lib.jsd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, vp,
                                      ctypes.POINTER(vp)]
     lib.jsd_batch_destroy.argtypes = [vp]

@@ -1103,6 +1103,39 @@ __global__ __launch_bounds__(256) void k_nl_lin(NlLinArgs A) {
   nl_lin_body(A, Blk{(int)blockIdx.x, 0, 0, (int)gridDim.x});
 }
 
+// The h_g addend of coupling_term = False (navier_stokes_perturbation.py:
+// 214-224): G = (i kz) hat(U'(y) no_hat(v)).  The transform pair about a
+// y-only factor is U'(y) times the Hermitian part of v with the reference's
+// +-K storage map (as in k_nl_lin); the z-derivative uses the stored kz.
+__global__ __launch_bounds__(256) void k_coupling_corr(const double2* __restrict__ V, const double* __restrict__ dUb,
+                                                      const double* __restrict__ kz, double2* __restrict__ G, int M,
+                                                      int N, int Z) {
+  const size_t tot = (size_t)M * N * Z;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= tot) return;
+  const int j = e % Z;
+  const int l = (e / Z) % N;
+  const int i = e / ((size_t)Z * N);
+  const int K = M / 2, KZ = Z / 2;
+  int si = i, sj = j;
+  if (i == K + 1 && j == KZ + 1) {
+    si = K;
+    sj = KZ;
+  } else if (i == K + 1) {
+    sj = (Z - j) % Z;
+  } else if (j == KZ + 1) {
+    si = (M - i) % M;
+  }
+  const int pi = (M - si) % M, pj = (Z - sj) % Z;
+  const double2 hv = cherm(V[((size_t)si * N + l) * Z + sj], V[((size_t)pi * N + l) * Z + pj]);
+  G[e] = cik(kz[j], cscale(hv, dUb[l]));
+}
+
+__global__ __launch_bounds__(256) void k_add(const double2* __restrict__ A, double2* __restrict__ B, size_t n) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) B[e] = cadd(B[e], A[e]);
+}
+
 // ===========================================================================
 // per-wavenumber implicit solve (navier_stokes.py:1248-1546)
 // ===========================================================================
@@ -1262,6 +1295,7 @@ struct jsd_plan {
   double *EVF, *EIF, *LAM;  // eigen-form solve: E, E^-1 fragments [even, odd], eigenvalues [2][PH]
   double *FF, *GF, *CWE, *RWE;  // E^-1 D and D E fragments, wall columns / rows in eigen coordinates
   double *VD2, *Dm;             // D^2 v_a parity tables with wall parts; D row-major (mean-flow derivatives)
+  double2* HGC;                 // coupling_term = False: the h_g addend of the current substep
   double c_impl[3];         // beta_s dt / Re of the Helmholtz operators
   double *EH_R, *EG_R, *LAMR;  // adjoint eigen form: full-row E and E^-1 (row-major), eigenvalue per row
   double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
@@ -1931,6 +1965,14 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   TRY(make_fft(p, d->ppz, &p->fpz, &p->permz, &p->ipermz, &p->twz));
   const size_t fb = p->fs * sizeof(double2);
   TRY(dalloc(p, (void**)&p->DU, 3 * fb));
+  p->HGC = nullptr;
+  if (d->no_coupling_term) {
+    if (d->kind != JSD_KIND_PERTURBATION) {
+      jsd_plan_destroy(p);
+      JSD_FAIL(JSD_ERR_ARG, "no_coupling_term applies to perturbation plans");
+    }
+    TRY(dalloc(p, (void**)&p->HGC, fb));
+  }
   TRY(dalloc(p, (void**)&p->H, 3 * fb));
   TRY(dalloc(p, (void**)&p->HV, fb));
   TRY(dalloc(p, (void**)&p->HG, fb));
@@ -2046,6 +2088,11 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   const size_t fs = p->fs;
   const int M = p->M, N = p->N, Z = p->Z;
   const bool lin = (d.kind != JSD_KIND_NS) && d.linearise;
+  if (p->HGC) {  // coupling_term = False: the h_g addend d_z hat(U' v)
+    hipLaunchKernelGGL(k_coupling_corr, dim3((unsigned)((fs + 255) / 256)), dim3(256), 0, st, U + fs, p->dUb, p->kz,
+                       p->HGC, M, N, Z);
+    HIPCHK(hipGetLastError());
+  }
   if (lin) {
     int rc = have_du ? JSD_OK : launch_ygemm_plain(p, U + fs, p->DU, 1, 1, st);
     if (rc) return rc;
@@ -2211,6 +2258,7 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
   S.Dm = p->Dm;
   S.DUo = nullptr;
   S.du_full = 0;
+  S.HGC = p->HGC;
   S.c_impl = p->EVF ? p->c_impl[s] : 0.0;
   S.CW = p->CW;
   S.RW = p->RW;
@@ -2310,6 +2358,11 @@ extern "C" int jsd_nonlinear(jsd_plan* p, const void* vel_hat, const void* fwd_h
   if (rc) return rc;
   rc = launch_hv(p, p->H, (double2*)h_v, (double2*)h_g, p->CONV, st);
   if (rc) return rc;
+  if (p->HGC) {
+    hipLaunchKernelGGL(k_add, dim3((unsigned)((p->fs + 255) / 256)), dim3(256), 0, st, (const double2*)p->HGC,
+                       (double2*)h_g, p->fs);
+    HIPCHK(hipGetLastError());
+  }
   // conv_ns = -H
   const size_t n = 6 * p->fs;
   hipLaunchKernelGGL(k_negate, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)p->H, (double*)conv_ns, n);
@@ -2354,6 +2407,17 @@ extern "C" int jsd_profile(jsd_plan* p, int enable, double* ms, long* count) {
   }
   for (int c = 0; c < JSD_NPROF; ++c) p->prof_used[c] = 0;
   p->prof_on = enable & ((1 << JSD_NPROF) - 1);
+  return JSD_OK;
+}
+
+extern "C" int jsd_set_base_flow(jsd_plan* p, const double* base_u, const double* base_du, void* stream) {
+  if (!p || !base_u || !base_du) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (p->d.kind == JSD_KIND_NS || !p->Ub) JSD_FAIL(JSD_ERR_STATE, "the plan has no base flow");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t b = (size_t)p->N * sizeof(double);
+  HIPCHK(hipMemcpyAsync(p->Ub, base_u, b, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(p->dUb, base_du, b, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host arrays may go away after return
   return JSD_OK;
 }
 
@@ -2556,6 +2620,7 @@ extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, do
   if (p->d.kind == JSD_KIND_PERTURBATION && p->d.linearise) return jsd_step_vjp(p, grad, 1, stream);
   if (p->d.kind == JSD_KIND_DUAL || !p->adjoint)
     JSD_FAIL(JSD_ERR_STATE, "the nonlinear adjoint needs an NS or perturbation plan created with adjoint = 1");
+  if (p->HGC) JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint has no coupling_term = False transpose");
   hipStream_t st = (hipStream_t)stream;
   const size_t sb = 3 * p->fs * sizeof(double2);
   if (!p->SS) {
@@ -2585,6 +2650,7 @@ extern "C" int jsd_step_vjp(jsd_plan* p, void* grad, int nsteps, void* stream) {
   if (!p || !grad || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   if (p->d.kind != JSD_KIND_PERTURBATION || !p->d.linearise || !p->adjoint)
     JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint is built for linearised perturbation plans");
+  if (p->HGC) JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint has no coupling_term = False transpose");
   hipStream_t st = (hipStream_t)stream;
   for (int n = 0; n < nsteps; ++n)
     for (int s = 2; s >= 0; --s) {
@@ -2833,9 +2899,9 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_ARG, "run %d: missing plan or buffer", r);
     }
-    if (!p->d.linearise || p->d.kind == JSD_KIND_NS) {
+    if (!p->d.linearise || p->d.kind == JSD_KIND_NS || p->HGC) {
       jsd_batch_destroy(b);
-      JSD_FAIL(JSD_ERR_STATE, "run %d: batches hold linearised perturbation or dual plans only", r);
+      JSD_FAIL(JSD_ERR_STATE, "run %d: batches hold linearised perturbation or dual plans (coupling term on) only", r);
     }
     if (b->vjp && (p->d.kind != JSD_KIND_PERTURBATION || !p->adjoint)) {
       jsd_batch_destroy(b);

@@ -10,6 +10,7 @@ from typing import Any
 from . import _lib
 from .navier_stokes import NavierStokesVelVort
 from .field import FourierField, PhysicalField, VectorField
+from .plan import StepPlan
 
 
 class NavierStokesVelVortPerturbation(NavierStokesVelVort):
@@ -20,12 +21,11 @@ class NavierStokesVelVortPerturbation(NavierStokesVelVort):
         super().__init__(velocity_field, **params)
         if params.get("linearise_switch") is not None or params.get("coupling_term_switch") is not None:
             raise NotImplementedError("time-dependent linearise / coupling switches are not part of the MI355X step")
-        if not params.get("coupling_term", True):
-            raise NotImplementedError("coupling_term=False is not part of the MI355X step")
         if "velocity_base_hat" in params:
             raise NotImplementedError("the MI355X step uses the laminar base flow u_max (1 - y^2)")
         self.linearise_flag = bool(params.get("linearise", False))
-        self.coupling_term_flag = True
+        # coupling_term = False adds d_z hat(U' v) to h_g (navier_stokes_perturbation.py:214-224)
+        self.coupling_term_flag = bool(params.get("coupling_term", True))
         self.dPdx = 0.0
         pd = self.physical_domain
         um = self.u_max_over_u_tau
@@ -38,4 +38,13 @@ class NavierStokesVelVortPerturbation(NavierStokesVelVort):
         return self.linearise_flag
 
     def coupling_term(self, _t: int = 0) -> bool:
-        return True
+        return self.coupling_term_flag
+
+    def plan(self) -> StepPlan:
+        if self._plan is None:
+            g = self.physical_domain.grid_math
+            self._plan = StepPlan(
+                g, self.Re_tau, self.get_dt(), self.kind, self.linearise_flag, self.u_max_over_u_tau,
+                coupling_term=self.coupling_term_flag,
+            )
+        return self._plan

@@ -50,6 +50,7 @@ class StepPlan:
         plan_re: Optional[float] = None,
         adjoint: bool = False,
         eigen_solve: bool = True,
+        coupling_term: bool = True,
     ):
         if not torch.cuda.is_available():
             raise _lib.JsdError("no GPU visible: the channel-flow step runs only through the HIP library")
@@ -95,6 +96,8 @@ class StepPlan:
         # eigen form of the implicit solve (shared E, E^-1 per parity half
         # instead of per-|k| inverse tables); False keeps the tables
         desc.y_eig = _ptr(arr(o.y_eig)) if eigen_solve else None
+        # coupling_term = False adds d_z hat(U' v) to h_g (navier_stokes_perturbation.py:214-224)
+        desc.no_coupling_term = 0 if coupling_term else 1
         self.has_adjoint = bool(adjoint) or (kind == _lib.KIND_PERTURBATION and self.linearise)
         h = ctypes.c_void_p()
         _lib.check(lib.jsd_plan_create(ctypes.byref(desc), ctypes.byref(h)))
@@ -125,6 +128,15 @@ class StepPlan:
     @property
     def shape(self) -> Tuple[int, int, int, int]:
         return (3, self.grid.nxm, self.grid.ny, self.grid.nzm)
+
+    def set_base_flow(self, base_u: np.ndarray, base_du: Optional[np.ndarray] = None) -> None:
+        """Replace the base profile U(y) (and U'(y), by default D U) of a
+        perturbation plan before the next step."""
+        u = np.ascontiguousarray(base_u, dtype=np.float64)
+        du = np.ascontiguousarray(self.grid.D @ u if base_du is None else base_du, dtype=np.float64)
+        if u.shape != (self.grid.ny,) or du.shape != (self.grid.ny,):
+            raise ValueError("base profiles need %d values" % self.grid.ny)
+        _lib.check(_lib.load().jsd_set_base_flow(self._h, _ptr(u), _ptr(du), _stream()))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -703,6 +703,58 @@ def base_flow_hat(dom: Dom, u_max: float = 1.0) -> torch.Tensor:
     return torch.stack([field_hat(dom, ub), field_hat(dom, zero), field_hat(dom, zero)])
 
 
+def instationary_base_profile(
+    y: np.ndarray, t: float, re: float, kappa: float = 1.0, accelerate: bool = True, n_max: int = 100
+) -> np.ndarray:
+    """Time-dependent base flow U(y, t) between walls moving with +-g_w(t)
+    (navier_stokes_perturbation_instationary.py:39-80, profile_fn):
+    g_w = 1 - exp(-kappa t) (accelerating) or exp(-kappa t), and
+    U = g_w y + Re/6 g_w'(t) (y^3 - y) + sum_{n=1}^{n_max-1} [(e^{-kappa t} - e^{-a_n t})
+    (-2 Re (-1)^n / (pi n)^3) kappa^2 s / (a_n - kappa) + e^{-a_n t} c_n] sin(n pi y)
+    with a_n = (n pi)^2 / Re, s = -1 (accelerating) or +1, c_n =
+    (-2 Re (-1)^n / (pi n)^3) g_w'(0); g_w' is the exact derivative (the
+    reference's jax.grad)."""
+    def g_w(tt):
+        return 1.0 - math.exp(-kappa * tt) if accelerate else math.exp(-kappa * tt)
+
+    def d_g_w(tt):
+        return kappa * math.exp(-kappa * tt) if accelerate else -kappa * math.exp(-kappa * tt)
+
+    out = g_w(t) * y + re / 6 * d_g_w(t) * (y**3 - y)
+    coeff = -1.0 if accelerate else 1.0
+    for n in range(1, n_max):
+        pre = -2 * re * (-1) ** n / (math.pi * n) ** 3
+        c_1_n = pre * d_g_w(0.0)
+        a_n = (math.pi * n) ** 2 / re
+        integ = kappa**2 * coeff / (a_n - kappa)
+        out = out + (math.exp(-kappa * t) - math.exp(-a_n * t)) * (pre * integ * np.sin(n * math.pi * y))
+        out = out + math.exp(-a_n * t) * c_1_n * np.sin(n * math.pi * y)
+    return out
+
+
+def make_perturbation_instationary(
+    dom: Dom, re: float, dt: float, kappa: float = 1.0, accelerate: bool = True, n_max: int = 100
+) -> StepParams:
+    """NavierStokesVelVortPerturbationInstationary
+    (navier_stokes_perturbation_instationary.py:29-108): the nonlinear
+    perturbation step about U(y, t) at t = time_step * dt (the same for the
+    three substeps, navier_stokes.py:1695), with coupling_term = False
+    (h_g gains d_z hat(U' v), navier_stokes_perturbation.py:214-224) and no
+    pressure gradient."""
+    ops = build_operators(dom, re, dt)
+    sp = StepParams(dom, ops, dt, 0.0)
+    X0, X1, X2 = np.meshgrid(dom.x, dom.y, dom.z, indexing="ij")
+    zero = field_hat(dom, torch.from_numpy(0.0 * X0 * X1 * X2))
+
+    def base(t: int) -> torch.Tensor:
+        U = instationary_base_profile(dom.y, t * dt, re, kappa, accelerate, n_max)
+        ub = torch.from_numpy(U[None, :, None] + 0.0 * X0 * X2)
+        return torch.stack([field_hat(dom, ub), zero, zero])
+
+    sp.nonlinear = lambda vh, t: nl_perturbation_skew(dom, vh, base(t), False, coupling_term=False)
+    return sp
+
+
 def make_ns(dom: Dom, re: float, dt: float, dPdx: float, u_max_over_u_tau: float = 1.0) -> StepParams:
     re_tau = re / u_max_over_u_tau
     ops = build_operators(dom, re_tau, dt)

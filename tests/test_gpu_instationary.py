@@ -1,0 +1,83 @@
+"""Perturbation steps about the time-dependent base flow with
+coupling_term = False (navier_stokes_perturbation_instationary.py:29-108)
+against the oracle, stage and step."""
+
+import numpy as np
+import pytest
+import torch
+
+from jax_spectral_dns_amd import KIND_PERTURBATION
+from jax_spectral_dns_amd.navier_stokes_perturbation_instationary import instationary_base_profile
+from jax_spectral_dns_amd.plan import StepPlan
+from oracle import jsd_oracle as O
+from tests.helpers import pair, perturbation_field, rel
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+@pytest.mark.parametrize("accelerate", [True, False])
+def test_instationary_nonlinear_term(accelerate):
+    g, d = pair((24, 33, 16), 2 * np.pi, np.pi)
+    re, dt, t = 500.0, 0.01, 7
+    plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=False, coupling_term=False)
+    U = instationary_base_profile(g.y, t * dt, re, 1.0, accelerate, 100)
+    plan.set_base_flow(U)
+    u = perturbation_field(g, seed=41, energy=1e-2)
+    hv, hg, conv = plan.nonlinear(u.to(DEV))
+    sp = O.make_perturbation_instationary(d, re, dt, 1.0, accelerate, 100)
+    rv, rg, _, rc = sp.nonlinear(u, t)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
+def test_instationary_steps():
+    g, d = pair((24, 33, 16), 2 * np.pi, np.pi)
+    re, dt = 500.0, 0.01
+    plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=False, coupling_term=False)
+    sp = O.make_perturbation_instationary(d, re, dt, 1.0, True, 100)
+    u = perturbation_field(g, seed=42, energy=1e-2)
+    ref, out = u.clone(), u.to(DEV).clone()
+    for k, n in enumerate(range(3, 7)):
+        ref = O.rk_step(sp, ref, n)
+        plan.set_base_flow(instationary_base_profile(g.y, n * dt, re, 1.0, True, 100))
+        plan.step(out, 0.0, 1)
+        err = rel(out, ref)
+        print("step", n, "rel", err)
+        assert err < 1e-11 * (k + 1)
+
+
+def test_coupling_correction_changes_only_h_g():
+    """coupling_term = False differs from the default only in h_g, by
+    d_z hat(U' v) (navier_stokes_perturbation.py:214-224)."""
+    g, d = pair((24, 33, 16), 2 * np.pi, np.pi)
+    u = perturbation_field(g, seed=43, energy=1e-2).to(DEV)
+    a = StepPlan(g, 1000.0, 0.01, KIND_PERTURBATION).nonlinear(u)
+    b = StepPlan(g, 1000.0, 0.01, KIND_PERTURBATION, coupling_term=False).nonlinear(u)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[2], b[2])
+    base = O.base_flow_hat(d)
+    comb = O.field_hat(d, O.diff_phys_y(d, O.field_no_hat(d, base[0])) * O.field_no_hat(d, u[1].cpu()))
+    assert rel(b[1] - a[1], O.diff(d, comb, 2)) < 1e-12
+
+
+def test_equation_class_solve_scan():
+    """NavierStokesVelVortPerturbationInstationary.solve_scan: four steps, each
+    with the base flow of its time step, against the oracle."""
+    from jax_spectral_dns_amd.domain import PhysicalDomain
+    from jax_spectral_dns_amd.field import FourierField, VectorField
+    from jax_spectral_dns_amd.navier_stokes_perturbation_instationary import NavierStokesVelVortPerturbationInstationary
+
+    shape, lx, lz = (24, 33, 16), 2 * np.pi, np.pi
+    pd = PhysicalDomain.create(shape, (True, False, True), scale_factors=(lx, 1.0, lz))
+    g, d = pair(shape, lx, lz)
+    u = perturbation_field(g, seed=44, energy=1e-2)
+    vh = VectorField.FromData(FourierField, pd, u.to(DEV), "velocity_hat")
+    nse = NavierStokesVelVortPerturbationInstationary(vh, Re=500.0, dt=0.01, end_time=0.04, kappa=1.0, accelerate=False)
+    out, _, n = nse.solve_scan()
+    assert n == 4
+    sp = O.make_perturbation_instationary(d, 500.0, 0.01, 1.0, False, 100)
+    ref = u.clone()
+    for k in range(n):
+        ref = O.rk_step(sp, ref, k)
+    assert rel(out[-1], ref) < 4e-11
