@@ -332,6 +332,163 @@ __global__ __launch_bounds__(256) void k_ypar(YparArgs G) {
   ypar_body(G, blockIdx.x, blockIdx.z, sO);
 }
 
+// Streaming form of the same product for the plain launches.  A persistent
+// grid of 512-thread workgroups stages the operator in LDS once, then every
+// wave walks its own sequence of 16-column units (flattened over components,
+// kx rows and kz re/im) with no further barrier.  A unit computes both output
+// halves from one read of its rows: with x_h, x_h' read for h < ceil(N/2),
+// e = (x_h + x_h')/2 and o = (x_h - x_h')/2 feed the even and odd blocks
+// (0.5 (x + x) = x exactly at the centre row, where o = 0).
+//
+// Operator layout (YsArgs::F, built by ystream_frags): for each of the KL
+// k-steps, 8 MFMA fragments (even-output tiles 0..3, odd-output tiles 0..3,
+// rows 0..63, zero where a block has no such tile or k-step), then after all
+// k-steps the coefficients of row 64 of each half (Ny = 129, 130: a VALU dot
+// product whose four per-row-group partial sums meet by lane shuffles instead
+// of a fifth tile of 15/16 padding), [ks][half][kr].
+//
+// The k-steps of a wave's units form one stream: the row pairs YS_PD k-steps
+// ahead, across unit boundaries, are in flight while the current k-step's
+// MFMAs run, so HBM reads overlap the products.
+#define YS_PD 4
+#define YS_THREADS 512
+
+struct YsArgs {
+  const double* F;  // ystream_frags layout
+  int KL;           // k-steps of a unit: ceil(ceil(N/2) / 4)
+  int nEf, nOf;     // half sizes
+  int xE, xO;       // row 64 present in the even / odd output half
+  int anti;
+  const double* in;
+  double* out;
+  int M, N, Z;
+  int comp_stride;  // doubles between components
+  int ncol;         // flattened columns over all components
+  const double* zero;  // one 0.0 in device memory (the read of a lane outside the data)
+};
+
+__host__ __device__ inline size_t ystream_lds_doubles(int KL) { return (size_t)KL * (8 * 64 + 8); }
+
+// FULL: ceil(N/2) >= 64, every tile row is a stored row (the bench sizes).
+// The loop runs whole groups of YS_PD k-steps (the ring slots are fixed
+// registers), rounded up: the extra positions read zeros and store nothing.
+template <bool ANTI, bool FULL>
+__global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
+  extern __shared__ double sF[];
+  const int KL = G.KL;
+  const int nlds = (int)ystream_lds_doubles(KL);
+  for (int w = threadIdx.x; w < nlds; w += YS_THREADS) sF[w] = G.F[w];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kr = lane >> 4, cc = lane & 15;
+  const int N = G.N, C2 = 2 * G.Z, nEf = G.nEf, nOf = G.nOf, ncol = G.ncol;
+  const int MC2 = G.M * C2;
+  const int nunit = (ncol + 15) / 16;
+  const int W = gridDim.x * (YS_THREADS / 64);
+  const int u0 = blockIdx.x * (YS_THREADS / 64) + wave;
+  const double* __restrict__ in = G.in;
+  double* __restrict__ out = G.out;
+  const double* __restrict__ sC = sF + (size_t)KL * 8 * 64 + kr;
+  // element offset of this lane's column in unit u (its y = +1 row), or -1
+  auto col_base = [&](int u) -> int {
+    const int q = u * 16 + cc;
+    if (u >= nunit || q >= ncol) return -1;
+    const int comp = q / MC2, rem = q - comp * MC2;
+    const int i = rem / C2;
+    return comp * G.comp_stride + i * N * C2 + (rem - i * C2);
+  };
+  // rows 4 ks + kr and N-1-(4 ks + kr).  Branch-free: a lane outside the
+  // data reads a zero word instead, so every position issues the same two
+  // loads.  dep (the value just taken from the slot being refilled) enters
+  // the address through an empty asm, so the refill cannot be scheduled
+  // before that use and reuses the slot's registers: no copies of in-flight
+  // values at the loop edge, and the waits before the MFMAs leave the newer
+  // prefetches in flight.
+  auto fetch = [&](int base, int ks, double dep, double& xa, double& xb) {
+    const bool v = base >= 0 && 4 * ks + kr < nEf;
+    int oa = base + kr * C2 + 4 * ks * C2, ob = base + (N - 1 - kr) * C2 - 4 * ks * C2;
+    __asm__("" : "+v"(oa), "+v"(ob) : "v"(dep));
+    xa = *(v ? in + oa : G.zero);
+    xb = *(v ? in + ob : G.zero);
+  };
+  const int nu = u0 < nunit ? (nunit - u0 + W - 1) / W : 0;
+  const int npos = ((nu * KL + YS_PD - 1) / YS_PD) * YS_PD;
+  // prefetch cursor (unit base, k-step) and compute cursor
+  int fu = u0, fb = col_base(u0), fk = 0;
+  double ra[YS_PD], rb[YS_PD];
+#pragma unroll
+  for (int j = 0; j < YS_PD; ++j) {
+    fetch(fb, fk, 0.0, ra[j], rb[j]);
+    if (++fk == KL) {
+      fk = 0;
+      fu += W;
+      fb = col_base(fu);
+    }
+  }
+  int cu = u0, cb = col_base(u0), ck = 0;
+  dbl4 ae[4], ao[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) ae[t] = ao[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  double pe = 0.0, po = 0.0;
+  for (int p = 0; p < npos; p += YS_PD) {
+#pragma unroll
+    for (int j = 0; j < YS_PD; ++j) {
+      const double e = 0.5 * (ra[j] + rb[j]), o = 0.5 * (ra[j] - rb[j]);
+      fetch(fb, fk, o, ra[j], rb[j]);
+      if (++fk == KL) {
+        fk = 0;
+        fu += W;
+        fb = col_base(fu);
+      }
+      const double inE = ANTI ? o : e, inO = ANTI ? e : o;
+      const double* __restrict__ f = sF + (size_t)ck * 512 + lane;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ae[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[t * 64], inE, ae[t], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ao[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(f[(4 + t) * 64], inO, ao[t], 0, 0, 0);
+      pe = fma(sC[ck * 8], inE, pe);
+      po = fma(sC[ck * 8 + 4], inO, po);
+      if (++ck == KL) {
+        // unit done: rows 0..63 from the tiles, row 64 from the dot products
+        pe += __shfl_xor(pe, 16);
+        pe += __shfl_xor(pe, 32);
+        po += __shfl_xor(po, 16);
+        po += __shfl_xor(po, 32);
+        if (cb >= 0) {
+          // row h = 16 t + 4 r + kr (f64 16x16x4: reg r of lane L is row
+          // 4r + L/16) and its mirror N-1-h: lane pointers plus offsets that
+          // are the same for every lane
+          double* __restrict__ lp = out + cb + kr * C2;
+          double* __restrict__ lm = out + cb + (N - 1 - kr) * C2;
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int h0 = t * 16 + 4 * r, h = h0 + kr;
+              if (!FULL && h >= nEf) continue;
+              const double ev = ae[t][r];
+              const double od = h < nOf ? ao[t][r] : 0.0;
+              // the centre row (h = N-1-h) has od = 0: both stores agree
+              lp[h0 * C2] = ev + od;
+              lm[-h0 * C2] = ev - od;
+            }
+          if (G.xE && kr == 0) {
+            const double od = G.xO ? po : 0.0;
+            lp[64 * C2] = pe + od;
+            lm[-64 * C2] = pe - od;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ae[t] = ao[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        pe = po = 0.0;
+        ck = 0;
+        cu += W;
+        cb = col_base(cu);
+      }
+    }
+  }
+}
+
 // ===========================================================================
 // FFT passes (nonlinear path)
 // ===========================================================================
@@ -1288,6 +1445,10 @@ struct jsd_plan {
   double *DF1, *DF2, *DFT1, *D_R, *D2_R, *RB_R, *LI_R, *PO_R;  // adjoint: D^T tiles, row-major tables
   double *YF1, *YF2;  // parity-block fragments of D and D^2 for k_ypar
   YparArgs yp1, yp2;  // their block geometry
+  double *YS1, *YS2;  // the same operators in k_ypar_stream's layout
+  double* YSZ;        // zeros read by its lanes outside the data
+  YsArgs ys1, ys2;
+  int ys_grid[2];     // persistent grid of k_ypar_stream for D, D^2 (0: k_ypar instead)
   double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
   int adjoint;         // the discrete-adjoint tables are present
   double2* SS;         // adjoint of a nonlinear step: the three recomputed substep states
@@ -1688,6 +1849,81 @@ static int eigen_tables_full(jsd_plan* p, const jsd_plan_desc* d, int N) {
 
 // Fragments of the full-row parity blocks of A (walls included: half index h
 // is row h and its mirror N-1-h), sgn = -1 for D (parities swap), +1 for D^2.
+// k_ypar_stream's operator layout (see there) from the same parity-block
+// entries as ypar_frags.
+static void ystream_frags(const double* A, int N, int sgn, std::vector<double>& F, YsArgs& G) {
+  const int nEf = (N + 1) / 2, nOf = N / 2;
+  G.nEf = nEf;
+  G.nOf = nOf;
+  G.KL = (nEf + 3) / 4;
+  G.xE = nEf > 64;
+  G.xO = nOf > 64;
+  G.anti = sgn < 0 ? 1 : 0;
+  const int KL = G.KL;
+  F.assign(ystream_lds_doubles(KL), 0.0);
+  auto entry = [&](int h, int qi, int g) {
+    const int m = g, mp = N - 1 - g;
+    if (m == mp) return sym_entry(A, N, sgn, h, m);
+    return sym_entry(A, N, sgn, h, m) + (qi ? -1.0 : 1.0) * sym_entry(A, N, sgn, h, mp);
+  };
+  for (int qo = 0; qo < 2; ++qo) {
+    const int qi = G.anti ? 1 - qo : qo;
+    const int no = qo ? nOf : nEf, ni = qi ? nOf : nEf;
+    for (int ks = 0; ks < KL; ++ks) {
+      for (int t = 0; t < 4; ++t)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int h = t * 16 + (lane & 15), g = ks * 4 + (lane >> 4);
+          F[((size_t)ks * 8 + qo * 4 + t) * 64 + lane] = (h < no && g < ni) ? entry(h, qi, g) : 0.0;
+        }
+      for (int kr = 0; kr < 4; ++kr) {
+        const int g = ks * 4 + kr;
+        F[(size_t)KL * 512 + ks * 8 + qo * 4 + kr] = (64 < no && g < ni) ? entry(64, qi, g) : 0.0;
+      }
+    }
+  }
+}
+
+// Persistent grids of k_ypar_stream for D and D^2: resident workgroups per
+// CU at its LDS size times the CUs.  Zero (k_ypar runs instead) for Ny < 25
+// (fewer k-steps than the prefetch distance), Ny > 130, or when the device
+// cannot hold the operator in LDS.
+static int ypar_stream_setup(jsd_plan* p, const double* D, const double* D2) {
+  p->ys_grid[0] = p->ys_grid[1] = 0;
+  const int N = p->N;
+  if ((N + 1) / 2 > 65) return JSD_OK;
+  std::vector<double> f;
+  ystream_frags(D, N, -1, f, p->ys1);
+  int rc = upload(p, &p->YS1, f.data(), f.size());
+  if (rc) return rc;
+  ystream_frags(D2, N, +1, f, p->ys2);
+  rc = upload(p, &p->YS2, f.data(), f.size());
+  if (rc) return rc;
+  const double z8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  rc = upload(p, &p->YSZ, z8, 8);
+  if (rc) return rc;
+  if (p->ys1.KL < YS_PD) return JSD_OK;
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const size_t sh = ystream_lds_doubles(p->ys1.KL) * sizeof(double);
+  for (int o = 0; o < 2; ++o) {
+    const bool full = p->ys1.nEf >= 64;
+    const void* fn = o == 0 ? (full ? (const void*)k_ypar_stream<true, true> : (const void*)k_ypar_stream<true, false>)
+                            : (full ? (const void*)k_ypar_stream<false, true> : (const void*)k_ypar_stream<false, false>);
+    if (sh > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, YS_THREADS, sh) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (nb > 0 && ncu > 0) p->ys_grid[o] = nb * ncu;
+  }
+  return JSD_OK;
+}
+
 static void ypar_frags(const double* A, int N, int sgn, std::vector<double>& F, YparArgs& G) {
   const int nEf = (N + 1) / 2, nOf = N / 2;
   G.nEf = nEf;
@@ -1792,6 +2028,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   TRY(upload(p, &p->YF1, tiled.data(), tiled.size()));
   ypar_frags(D2.data(), N, +1, tiled, p->yp2);
   TRY(upload(p, &p->YF2, tiled.data(), tiled.size()));
+  TRY(ypar_stream_setup(p, d->D, D2.data()));
   {
     // parity blocks: D (centro-antisymmetric), D^2, rhs_base[s], lhs_inv[s][k],
     // poisson[k] (centro-symmetric), wall rows/columns, v_a in parity form
@@ -2048,8 +2285,34 @@ static int launch_ygemm_plain(jsd_plan* p, const double2* in, double2* out, int 
   unsigned nblk = 0;
   size_t lds = 0;
   const YparArgs G = ypar_args(p, in, out, order, &nblk, &lds);
+  const int o = order == 2 ? 1 : 0;
   prof_begin(p, 0, st);
-  hipLaunchKernelGGL(k_ypar, dim3(nblk, 1, ncomp), dim3(256), lds, st, G);
+  if (p->ys_grid[o] > 0 && (long)ncomp * G.comp_stride < (1L << 31)) {
+    YsArgs Y = o ? p->ys2 : p->ys1;
+    Y.F = o ? p->YS2 : p->YS1;
+    Y.in = G.in;
+    Y.out = G.out;
+    Y.M = p->M;
+    Y.N = p->N;
+    Y.Z = p->Z;
+    Y.comp_stride = (int)G.comp_stride;
+    Y.ncol = ncomp * p->M * 2 * p->Z;
+    Y.zero = p->YSZ;
+    const int units = (Y.ncol + 15) / 16;
+    const int grid = std::min(p->ys_grid[o], (units + YS_THREADS / 64 - 1) / (YS_THREADS / 64));
+    const size_t sh = ystream_lds_doubles(Y.KL) * sizeof(double);
+    const bool full = Y.nEf >= 64;
+    if (Y.anti && full)
+      hipLaunchKernelGGL((k_ypar_stream<true, true>), dim3(grid), dim3(YS_THREADS), sh, st, Y);
+    else if (Y.anti)
+      hipLaunchKernelGGL((k_ypar_stream<true, false>), dim3(grid), dim3(YS_THREADS), sh, st, Y);
+    else if (full)
+      hipLaunchKernelGGL((k_ypar_stream<false, true>), dim3(grid), dim3(YS_THREADS), sh, st, Y);
+    else
+      hipLaunchKernelGGL((k_ypar_stream<false, false>), dim3(grid), dim3(YS_THREADS), sh, st, Y);
+  } else {
+    hipLaunchKernelGGL(k_ypar, dim3(nblk, 1, ncomp), dim3(256), lds, st, G);
+  }
   prof_end(p, 0, st);
   HIPCHK(hipGetLastError());
   return JSD_OK;
