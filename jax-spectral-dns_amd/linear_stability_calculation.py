@@ -85,6 +85,7 @@ class LinearStabilityCalculation:
         self.eigenvectors: Optional[np.ndarray] = None  # columns [v; eta], sorted like eigenvalues
         self.S: Optional[np.ndarray] = None
         self.coeffs: Optional[np.ndarray] = None
+        self.velocity_field_ = None
 
     @property
     def k2(self) -> float:
@@ -178,6 +179,80 @@ class LinearStabilityCalculation:
         q = self.eigenvectors[:, : len(coeffs)] @ coeffs
         return self._velocity(q)
 
+    # ---- fields on a DNS grid -------------------------------------------
+    def _check_grid(self, domain) -> None:
+        if len(domain.grid[1]) != self.n:
+            raise ValueError("the stability calculation must use the domain's Ny = %d points" % len(domain.grid[1]))
+
+    def y_slice_to_3d_field(self, domain, y_slice_i: np.ndarray, factor: complex = 1.0) -> np.ndarray:
+        """Re{factor * q(y) exp(i(alpha x + beta z))} on the domain's physical
+        grid (linear_stability_calculation.py:313-335)."""
+        self._check_grid(domain)
+        x, z = np.asarray(domain.grid[0]), np.asarray(domain.grid[2])
+        phase = np.exp(1j * self.alpha * x)[:, None, None] * np.exp(1j * self.beta * z)[None, None, :]
+        return (factor * phase * np.asarray(y_slice_i)[None, :, None]).real
+
+    def velocity_field_from_y_slice(self, domain, y_slice, factor: complex = 1.0):
+        """Physical VectorField of the three profiles
+        (linear_stability_calculation.py:337-364)."""
+        import torch
+
+        from .field import PhysicalField, VectorField
+
+        names = ("velocity_pert_x", "velocity_pert_y", "velocity_pert_z")
+        comps = [
+            PhysicalField(domain, torch.from_numpy(self.y_slice_to_3d_field(domain, q, factor)), name=nm)
+            for q, nm in zip(y_slice, names)
+        ]
+        self.velocity_field_ = VectorField(comps)
+        return self.velocity_field_
+
+    def velocity_field(self, domain, evec: np.ndarray, factor: complex = 1.0):
+        """Physical VectorField of one eigenvector (linear_stability_calculation.py:268-311).
+        Here evec is [v; eta] on the collocation points (the reference's is
+        the primitive (u, v, w, p) coefficient vector of its own basis)."""
+        return self.velocity_field_from_y_slice(domain, self._velocity(np.asarray(evec)), factor)
+
+    def velocity_field_single_mode(
+        self,
+        domain,
+        mode: int = 0,
+        factor: complex = 1.0,
+        recompute_partial: bool = False,
+        recompute_full: bool = False,
+        save: bool = False,
+    ):
+        """Physical VectorField of eigenmode `mode` (0: least stable), e.g. a
+        Tollmien-Schlichting wave (linear_stability_calculation.py:225-266).
+        The reference first tries fields cached on disk; this computes them
+        (recompute_partial / recompute_full accepted for the same call
+        sites).  save writes the three components as field files
+        (u/v/w_Re_<Re>_n_<n>_mode_<mode>, field.save_to_file)."""
+        if recompute_full or self.eigenvalues is None:
+            self.calculate_eigenvalues()
+        u = self.velocity_field(domain, self.eigenvectors[:, mode], factor)
+        if save:
+            for c, comp in zip("uvw", u):
+                comp.save_to_file("%s_Re_%s_n_%d_mode_%d" % (c, self.Re, self.n, mode))
+        return u
+
+    def energy_over_time(self, domain, mode: int = 0, eps: float = 1.0):
+        """(E(t, dim=None), lambda): the linear theory's energy of eps times
+        eigenmode `mode` at time t, E(0) exp(2 Re(lambda) t), all components or
+        one (linear_stability_calculation.py:366-422)."""
+        if self.eigenvalues is None:
+            self.calculate_eigenvalues()
+        if getattr(self, "velocity_field_", None) is None:
+            self.velocity_field(domain, self.eigenvectors[:, mode])
+        lam = self.eigenvalues[mode]
+        comp_energy = [float(c.energy()) for c in self.velocity_field_]
+
+        def out(t: float, dim: Optional[int] = None) -> float:
+            e = sum(comp_energy) if dim is None else comp_energy[dim]
+            return float(eps**2 * e * np.exp(2.0 * lam.real * t))
+
+        return out, lam
+
     def calculate_transient_growth_initial_condition(
         self, domain, T: float, number_of_modes: int, energy: Optional[float] = None
     ):
@@ -188,12 +263,9 @@ class LinearStabilityCalculation:
         With energy, the field is scaled to that energy."""
         import torch
 
-        if len(domain.grid[1]) != self.n:
-            raise ValueError("the stability calculation must use the domain's Ny = %d points" % len(domain.grid[1]))
+        self._check_grid(domain)
         u, v, w = self.optimal_profiles(T, number_of_modes)
-        x, z = domain.grid[0], domain.grid[2]
-        phase = np.exp(1j * self.alpha * x)[:, None, None] * np.exp(1j * self.beta * z)[None, None, :]
-        phys = np.stack([(phase * c[None, :, None]).real for c in (u, v, w)])
+        phys = np.stack([self.y_slice_to_3d_field(domain, c) for c in (u, v, w)])
         hat = torch.stack([domain.field_hat(torch.from_numpy(phys[c])) for c in range(3)])
         if energy is not None:
             from .initial import spectral_energy

@@ -64,3 +64,42 @@ def test_optimal_initial_condition_is_divergence_free_and_scaled():
     nz = [tuple(ij) for ij in torch.nonzero(mags > 1e-12 * mags.max()).tolist()]
     M, Z = mags.shape
     assert sorted(nz) == [(1, 1), (M - 1, Z - 1)]  # (alpha, beta) and its conjugate only
+
+
+def test_neutral_point_of_plane_poiseuille():
+    """Critical point of plane Poiseuille flow, Re = 5772.22, alpha = 1.02056:
+    neutral least stable mode with phase speed c_r = 0.26400174 (Orszag 1971),
+    the case of the reference's Tollmien-Schlichting growth-rate test
+    (tests/test_project.py:1320-1500)."""
+    lsc = LinearStabilityCalculation(Re=5772.22, alpha=1.02056, beta=0.0, n=100)
+    lam, _ = lsc.calculate_eigenvalues()
+    c = 1j * lam[0] / lsc.alpha
+    assert abs(c.real - 0.26400174) < 1e-7
+    assert abs(lam[0].real) < 1e-7
+
+
+def test_single_mode_field_and_energy_over_time():
+    """velocity_field_single_mode: one (alpha, 0) wave on the DNS grid,
+    divergence free, no-slip; energy_over_time(0) is its energy and the
+    energy grows as exp(2 Re(lambda) t)."""
+    shape = (4, 33, 4)
+    dom = PhysicalDomain.create(shape, (True, False, True), scale_factors=(2 * np.pi / 1.02056, 1.0, 1e-3))
+    lsc = LinearStabilityCalculation(Re=6000.0, alpha=1.02056, beta=0.0, n=33)
+    u = lsc.velocity_field_single_mode(dom)
+    phys = torch.stack([c.get_data() for c in u])
+    assert phys.shape == (3,) + tuple(dom.get_shape_aliasing())
+    # walls
+    assert float(phys[:, :, 0].abs().max()) < 1e-12 * float(phys.abs().max())
+    assert float(phys[:, :, -1].abs().max()) < 1e-12 * float(phys.abs().max())
+    hat = torch.stack([dom.field_hat(phys[c]) for c in range(3)])
+    d = O.make_domain(shape, 2 * np.pi / 1.02056, 1e-3)
+    div = O.diff(d, hat[0], 0, 1) + O.diff(d, hat[1], 1, 1) + O.diff(d, hat[2], 2, 1)
+    assert torch.linalg.norm(div) < 1e-9 * torch.linalg.norm(hat)
+    mags = torch.linalg.norm(hat, dim=(0, 2))
+    nz = sorted(tuple(ij) for ij in torch.nonzero(mags > 1e-12 * mags.max()).tolist())
+    assert nz == [(1, 0), (hat.shape[1] - 1, 0)]
+    fn, lam = lsc.energy_over_time(dom, eps=0.1)
+    e0 = sum(float(c.energy()) for c in u)
+    assert abs(fn(0.0) - 0.01 * e0) < 1e-14 * e0
+    assert abs(fn(2.0) / fn(0.0) - np.exp(4.0 * lam.real)) < 1e-14
+    assert abs(float(O.energy(d, hat)) - e0) < 1e-12 * e0

@@ -29,3 +29,33 @@ def test_linearised_step_attains_optimal_transient_growth():
     print("DNS gain %.6f, stability calculation G(T) %.6f" % (gain, G))
     assert abs(G - 20.36) < 0.02
     assert abs(gain - G) < 2e-3 * G
+
+
+@pytest.mark.parametrize("re", [5500.0, 5772.22, 6000.0])
+def test_tollmien_schlichting_growth_rate(re):
+    """The reference's Tollmien-Schlichting test (tests/test_project.py:1320-1500,
+    examples.py:624-760): the least stable mode at alpha = 1.02056 on a
+    quasi-2D box (Lz = 1e-3), Ny = 50, advanced by the linearised step for
+    t = 1 (dt = 0.02); its energy growth rate over that time matches the
+    linear theory's to 1e-3 relative at Re = 5500 and 6000, and is below
+    2e-6 at the critical Re = 5772.22 (neutral)."""
+    import torch
+
+    alpha, dt, T, ny = 1.02056, 0.02, 1.0, 50
+    dom = PhysicalDomain.create((4, ny, 4), (True, False, True), scale_factors=(2 * np.pi / alpha, 1.0, 1e-3))
+    lsc = LinearStabilityCalculation(Re=re, alpha=alpha, beta=0.0, n=ny)
+    u = lsc.velocity_field_single_mode(dom)
+    u_hat = torch.stack([dom.field_hat(c.get_data()) for c in u]).to("cuda")
+    energy_ana, _ = lsc.energy_over_time(dom)
+    plan = StepPlan(dom.grid_math, re, dt, KIND_PERTURBATION, linearise=True)
+    e0 = plan.energy(u_hat)
+    plan.step(u_hat, 0.0, int(round(T / dt)))
+    e1 = plan.energy(u_hat)
+    rate = (e1 - e0) / (T * energy_ana(0.0))
+    rate_ana = (energy_ana(T) - energy_ana(0.0)) / (T * energy_ana(0.0))
+    print("Re %.2f: growth rate %.6e, linear theory %.6e" % (re, rate, rate_ana))
+    assert abs(e0 - energy_ana(0.0)) < 1e-12 * e0
+    if re == 5772.22:
+        assert abs(rate) < 2e-6
+    else:
+        assert abs((rate_ana - rate) / (0.5 * (rate_ana + rate))) < 1e-3
