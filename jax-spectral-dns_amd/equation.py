@@ -3,8 +3,22 @@
 
 from __future__ import annotations
 
+import os
 import time
 from typing import Any, Callable, Dict, List, Optional
+
+import numpy as np
+
+
+def write_trajectory(trajectory: Any, filename: str = "trajectory") -> None:
+    """<field_dir>/<filename> holding the stacked states as one gzip-9 dataset
+    "trajectory" (equation.py:362-372, navier_stokes_perturbation_dual.py:877-889)."""
+    from . import hdf5
+    from .field import Field
+
+    os.makedirs(Field.field_dir, exist_ok=True)
+    data = trajectory.detach().cpu().numpy() if hasattr(trajectory, "detach") else np.asarray(trajectory)
+    hdf5.write(Field.field_dir + "/" + filename, {"trajectory": hdf5.Dataset(data)}, compression=9)
 
 
 class Equation:
@@ -95,12 +109,17 @@ class Equation:
         pass
 
     def solve(self) -> Any:
-        """jit mode: solve_scan; eager mode: step by step with the hooks."""
+        """jit mode: solve_scan; eager mode: step by step with the hooks.
+        With JAX_SPECTRAL_DNS_FIELD_DIR set, solve_scan's trajectory is written
+        to <field_dir>/trajectory as one gzip-9 dataset "trajectory"
+        (equation.py:362-372)."""
         self.prepare()
         if self.activate_jit_:
             t0 = time.time()
             out = self.solve_scan()  # type: ignore[attr-defined]
             self.last_solve_seconds = time.time() - t0
+            if os.environ.get("JAX_SPECTRAL_DNS_FIELD_DIR") is not None:
+                write_trajectory(out[0])
             return out
         while not self.done():
             self.before_time_step()

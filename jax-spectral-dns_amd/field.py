@@ -3,24 +3,55 @@
 Fields wrap torch tensors (complex128 spectral, float64 physical) on any
 device.  ``PhysicalField.FromFunc`` / ``.hat()`` / ``.energy()``,
 ``FourierField.no_hat()`` and ``VectorField`` keep the reference's names and
-semantics (field.py:294-611, 1154-1473, 2764-3091); plotting and HDF5 I/O are
-not part of this build.
+semantics (field.py:294-611, 1154-1473, 2764-3091).  Field files are HDF5 in
+the reference's layout (save_to_file / FromFile, through the package's own
+reader and writer); plotting is not part of this build.
 """
 
 from __future__ import annotations
 
-from typing import Callable, Generic, List, Optional, Sequence, TypeVar, Union
+import os
+from typing import Callable, Generic, List, Optional, Sequence, Tuple, TypeVar, Union
 
 import numpy as np
 import torch
 
+from . import hdf5
 from .domain import FourierDomain, PhysicalDomain
 
 T = TypeVar("T", "PhysicalField", "FourierField")
 
 
+def _field_path(filename: str) -> str:
+    """A name not starting with "." or "/" lives in Field.field_dir
+    (field.py:153-160)."""
+    if filename[0] in "./":
+        return filename
+    os.makedirs(Field.field_dir, exist_ok=True)
+    return Field.field_dir + "/" + filename
+
+
+def _numpy(t: torch.Tensor) -> np.ndarray:
+    return t.detach().cpu().numpy()
+
+
+def _read_group(filename: str, name: str, time_step: int) -> Tuple[hdf5.Dataset, int]:
+    """Dataset `name` of group str(time_step); a negative time_step counts from
+    the last group in h5py's iteration order (field.py:409-431)."""
+    tree = hdf5.read(filename)
+    if time_step < 0:
+        time_step = int(hdf5.names(tree)[-1]) + time_step + 1
+    grp = tree.get(str(time_step))
+    assert isinstance(grp, dict), "time step %d not found, only %s found." % (time_step, hdf5.names(tree))
+    dset = grp.get(name)
+    assert isinstance(dset, hdf5.Dataset), "dataset %s not found, only %s found." % (name, hdf5.names(grp))
+    return dset, time_step
+
+
 class Field:
     """Common arithmetic of physical and spectral fields (field.py:81-291)."""
+
+    field_dir = os.environ.get("JAX_SPECTRAL_DNS_FIELD_DIR", "./fields/")
 
     def __init__(self, data: torch.Tensor, name: str = "field"):
         self.data = data
@@ -63,6 +94,16 @@ class Field:
         """norm / number of dofs (field.py:235-236)."""
         return float(torch.linalg.norm(self.data.reshape(-1)) / self.data.numel())
 
+    def save_to_hdf_file(self, filename: str) -> None:
+        """Group str(time_step) with a gzip-9 dataset named after the field and
+        its int64 attribute time_step (field.py:144-151)."""
+        ds = hdf5.Dataset(_numpy(self.data), {"time_step": np.int64(self.time_step)})
+        hdf5.write(filename, {str(self.get_time_step()): {self.name: ds}}, compression=9)
+
+    def save_to_file(self, filename: str) -> None:
+        """field.py:153-165."""
+        self.save_to_hdf_file(_field_path(filename))
+
 
 class PhysicalField(Field):
     def __init__(self, physical_domain: PhysicalDomain, data: torch.Tensor, name: str = "field"):
@@ -104,6 +145,22 @@ class PhysicalField(Field):
     def diff(self, direction: int, order: int = 1) -> "PhysicalField":
         return PhysicalField(self.physical_domain, self.physical_domain.diff(self.data, direction, order), self.name)
 
+    @classmethod
+    def read_hdf(cls, filename: str, name: str, time_step: int) -> torch.Tensor:
+        """field.py:1352-1373."""
+        return torch.from_numpy(np.array(_read_group(filename, name, time_step)[0].data))
+
+    @classmethod
+    def FromFile(
+        cls, domain: PhysicalDomain, filename: str, name: str = "field", time_step: int = -1, device=None
+    ) -> "PhysicalField":
+        """A field saved by save_to_file (field.py:1375-1420); its shape must be
+        the domain's."""
+        data = cls.read_hdf(_field_path(filename), name, time_step)
+        if tuple(data.shape) != tuple(domain.get_shape_aliasing()):
+            raise ValueError("field of shape %s does not match the domain %s" % (tuple(data.shape), domain.get_shape_aliasing()))
+        return cls(domain, data.to(device=device), name)
+
 
 class FourierField(Field):
     def __init__(self, physical_domain: PhysicalDomain, data: torch.Tensor, name: str = "field_hat"):
@@ -126,6 +183,15 @@ class FourierField(Field):
 
     def energy(self) -> float:
         return self.no_hat().energy()
+
+    @classmethod
+    def FromFile(
+        cls, domain: PhysicalDomain, filename: str, name: str = "field_hat", time_step: int = -1, device=None
+    ) -> "FourierField":
+        data = torch.from_numpy(np.array(_read_group(_field_path(filename), name, time_step)[0].data))
+        if tuple(data.shape) != tuple(domain.get_shape()):
+            raise ValueError("field of shape %s does not match the domain %s" % (tuple(data.shape), domain.get_shape()))
+        return cls(domain, data.to(device=device), name)
 
 
 class VectorField(Generic[T]):
@@ -194,6 +260,39 @@ class VectorField(Generic[T]):
 
     def __neg__(self):
         return VectorField([-a for a in self.elements], self.name)
+
+    def save_to_hdf_file(self, filename: str) -> None:
+        """Group str(time_step) with one gzip-9 dataset (components stacked on
+        axis 0) named after the field, no attribute (field.py:718-736)."""
+        hdf5.write(filename, {str(self.get_time_step()): {self.name: hdf5.Dataset(_numpy(self.get_data()))}}, compression=9)
+
+    def save_to_file(self, filename: str) -> None:
+        """field.py:740-762."""
+        self.save_to_hdf_file(_field_path(filename))
+
+    @classmethod
+    def read_hdf(cls, filename: str, name: str, time_step: int) -> Tuple[torch.Tensor, int]:
+        """field.py:409-431."""
+        dset, ts = _read_group(filename, name, time_step)
+        return torch.from_numpy(np.array(dset.data)), ts
+
+    @classmethod
+    def FromFile(
+        cls, domain: PhysicalDomain, filename: str, name: str = "field", time_step: int = -1, device=None
+    ) -> "VectorField":
+        """A vector field saved by save_to_file (field.py:433-470): physical
+        components when the stored arrays have the domain's physical shape,
+        spectral components when they have its mode shape."""
+        data, ts = cls.read_hdf(_field_path(filename), name, time_step)
+        data = data.to(device=device)
+        if tuple(data.shape[1:]) == tuple(domain.get_shape_aliasing()) and not data.is_complex():
+            out = cls.FromData(PhysicalField, domain, data, name)
+        elif tuple(data.shape[1:]) == tuple(domain.get_shape()):
+            out = cls.FromData(FourierField, domain, data, name)
+        else:
+            raise ValueError("field of shape %s does not match the domain" % (tuple(data.shape),))
+        out.set_time_step(ts)
+        return out
 
     def curl(self) -> "VectorField":
         fd = self.elements[0].fourier_domain  # type: ignore[union-attr]

@@ -14,11 +14,13 @@ The gain gradient is (u0 - v0) / E0 (navier_stokes_perturbation_dual.py:
 from __future__ import annotations
 
 import math
+import os
 from typing import Any, List, Optional, Tuple
 
 import torch
 
 from . import _lib
+from .equation import write_trajectory
 from .field import FourierField, VectorField
 from .navier_stokes import median_factor
 from .navier_stokes_perturbation import NavierStokesVelVortPerturbation
@@ -68,6 +70,8 @@ class NavierStokesVelVortPerturbationDual:
         self._checkpoints: List[torch.Tensor] = []
         self._history: List[torch.Tensor] = []
         self.gain: Optional[float] = None
+        # set by the optimisation drivers (gradient_descent_solver.py:547-629)
+        self.write_trajectory = False
 
     @classmethod
     def FromNavierStokesVelVortPerturbation(cls, nse: NavierStokesVelVortPerturbation, **params: Any):
@@ -97,6 +101,12 @@ class NavierStokesVelVortPerturbationDual:
                 if need_history:
                     self._checkpoints.append(u.clone())
         self.u_T = u
+        if os.environ.get("JAX_SPECTRAL_DNS_FIELD_DIR") is not None and self.write_trajectory:
+            # the forward states this run keeps: every step without
+            # checkpointing, else the initial and final states and the
+            # checkpoints (navier_stokes_perturbation_dual.py:877-889)
+            states = self._history or (self._checkpoints if len(self._checkpoints) > 1 else [self.velocity_u_hat_0, u])
+            write_trajectory(torch.stack(states))
 
     def _forward_state(self, t: int, chunk: List[torch.Tensor], start: int) -> Optional[torch.Tensor]:
         if self.linearise_flag:
