@@ -74,10 +74,13 @@ typedef struct jsd_plan_desc {
                                inverses are then applied as E diag(.) E^-1 with
                                beta_s |dt/Re| from dt and re_tau; checked against
                                lhs_inv / poisson at plan creation */
-  int no_coupling_term;     /* perturbation plans: nonzero adds d_z hat(U' v) to h_g, the
-                               reference's coupling_term = False
-                               (navier_stokes_perturbation.py:214-224), as the time-dependent
-                               base flow uses (navier_stokes_perturbation_instationary.py:95-108) */
+  int no_coupling_term;     /* the reference's coupling_term = False, as the time-dependent
+                               base flow uses (navier_stokes_perturbation_instationary.py:95-108,
+                               navier_stokes_perturbation_instationary_dual.py:96-119):
+                               perturbation plans add d_z hat(U' v) to h_g
+                               (navier_stokes_perturbation.py:214-224), dual plans add
+                               (d_x^2 + d_z^2) hat(U' v_x) to h_v
+                               (navier_stokes_perturbation_dual.py:269-278) */
 } jsd_plan_desc;
 
 /* Replaces NavierStokesVelVort.__init__ / assemble_matrices

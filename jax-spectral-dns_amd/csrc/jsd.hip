@@ -1264,6 +1264,11 @@ __global__ __launch_bounds__(256) void k_nl_lin(NlLinArgs A) {
 // 214-224): G = (i kz) hat(U'(y) no_hat(v)).  The transform pair about a
 // y-only factor is U'(y) times the Hermitian part of v with the reference's
 // +-K storage map (as in k_nl_lin); the z-derivative uses the stored kz.
+// DUAL = false: G = d_z hat(U' v) with V = v (the perturbation step's h_g
+// addend).  DUAL = true: G += hat(U' v_x) with V = v_x, G = H_1 (the dual
+// step's h_v addend (d_x^2 + d_z^2) hat(U' v_x), which the solve forms from
+// H_1).  hat(no_hat(.)) of the stored modes is the Hermitian map of cherm.
+template <bool DUAL>
 __global__ __launch_bounds__(256) void k_coupling_corr(const double2* __restrict__ V, const double* __restrict__ dUb,
                                                       const double* __restrict__ kz, double2* __restrict__ G, int M,
                                                       int N, int Z) {
@@ -1285,7 +1290,12 @@ __global__ __launch_bounds__(256) void k_coupling_corr(const double2* __restrict
   }
   const int pi = (M - si) % M, pj = (Z - sj) % Z;
   const double2 hv = cherm(V[((size_t)si * N + l) * Z + sj], V[((size_t)pi * N + l) * Z + pj]);
-  G[e] = cik(kz[j], cscale(hv, dUb[l]));
+  if (DUAL) {
+    const double2 c = cscale(hv, dUb[l]), g = G[e];
+    G[e] = make_double2(g.x + c.x, g.y + c.y);
+  } else {
+    G[e] = cik(kz[j], cscale(hv, dUb[l]));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_add(const double2* __restrict__ A, double2* __restrict__ B, size_t n) {
@@ -1457,6 +1467,7 @@ struct jsd_plan {
   double *FF, *GF, *CWE, *RWE;  // E^-1 D and D E fragments, wall columns / rows in eigen coordinates
   double *VD2, *Dm;             // D^2 v_a parity tables with wall parts; D row-major (mean-flow derivatives)
   double2* HGC;                 // coupling_term = False: the h_g addend of the current substep
+  int dual_cc;                  // dual plan, coupling_term = False: H_1 += hat(U' v_x) before the solve
   double c_impl[3];         // beta_s dt / Re of the Helmholtz operators
   double *EH_R, *EG_R, *LAMR;  // adjoint eigen form: full-row E and E^-1 (row-major), eigenvalue per row
   double *RNS, *LNS, *kx, *kz, *Ub, *dUb, *wy;
@@ -2203,12 +2214,16 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   const size_t fb = p->fs * sizeof(double2);
   TRY(dalloc(p, (void**)&p->DU, 3 * fb));
   p->HGC = nullptr;
+  p->dual_cc = 0;
   if (d->no_coupling_term) {
-    if (d->kind != JSD_KIND_PERTURBATION) {
+    if (d->kind == JSD_KIND_PERTURBATION) {
+      TRY(dalloc(p, (void**)&p->HGC, fb));
+    } else if (d->kind == JSD_KIND_DUAL) {
+      p->dual_cc = 1;
+    } else {
       jsd_plan_destroy(p);
-      JSD_FAIL(JSD_ERR_ARG, "no_coupling_term applies to perturbation plans");
+      JSD_FAIL(JSD_ERR_ARG, "no_coupling_term applies to perturbation and dual plans");
     }
-    TRY(dalloc(p, (void**)&p->HGC, fb));
   }
   TRY(dalloc(p, (void**)&p->H, 3 * fb));
   TRY(dalloc(p, (void**)&p->HV, fb));
@@ -2352,8 +2367,8 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   const int M = p->M, N = p->N, Z = p->Z;
   const bool lin = (d.kind != JSD_KIND_NS) && d.linearise;
   if (p->HGC) {  // coupling_term = False: the h_g addend d_z hat(U' v)
-    hipLaunchKernelGGL(k_coupling_corr, dim3((unsigned)((fs + 255) / 256)), dim3(256), 0, st, U + fs, p->dUb, p->kz,
-                       p->HGC, M, N, Z);
+    hipLaunchKernelGGL(k_coupling_corr<false>, dim3((unsigned)((fs + 255) / 256)), dim3(256), 0, st, U + fs, p->dUb,
+                       p->kz, p->HGC, M, N, Z);
     HIPCHK(hipGetLastError());
   }
   if (lin) {
@@ -2551,9 +2566,21 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
 
 // chain = true: p->DU already holds D U (the previous substep's solve wrote
 // it) and this substep's solve writes D of its new state for the next one.
+// Dual plans with coupling_term = False: H_1 += hat(U' v_x)
+// (navier_stokes_perturbation_dual.py:269-278).
+static int dual_coupling_corr(jsd_plan* p, const double2* U, hipStream_t st) {
+  if (!p->dual_cc) return JSD_OK;
+  hipLaunchKernelGGL(k_coupling_corr<true>, dim3((unsigned)((p->fs + 255) / 256)), dim3(256), 0, st, U, p->dUb,
+                     p->kz, p->H + p->fs, p->M, p->N, p->Z);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
 static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st,
                    bool have_du = false, bool write_du = false) {
   int rc = nonlinear_H(p, U, UF, st, have_du);
+  if (rc) return rc;
+  rc = dual_coupling_corr(p, U, st);
   if (rc) return rc;
   const int nblk = ((solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
   SolveArgs S = solve_args(p, U, s, dPdx, roles_of(p));
@@ -2619,6 +2646,13 @@ extern "C" int jsd_nonlinear(jsd_plan* p, const void* vel_hat, const void* fwd_h
   hipStream_t st = (hipStream_t)stream;
   int rc = nonlinear_H(p, (const double2*)vel_hat, (const double2*)fwd_hat, st, false);
   if (rc) return rc;
+  // conv_ns = -H, before the dual coupling correction (which the reference
+  // leaves out of conv_ns, navier_stokes_perturbation_dual.py:280-291)
+  const size_t n = 6 * p->fs;
+  hipLaunchKernelGGL(k_negate, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)p->H, (double*)conv_ns, n);
+  HIPCHK(hipGetLastError());
+  rc = dual_coupling_corr(p, (const double2*)vel_hat, st);
+  if (rc) return rc;
   rc = launch_hv(p, p->H, (double2*)h_v, (double2*)h_g, p->CONV, st);
   if (rc) return rc;
   if (p->HGC) {
@@ -2626,10 +2660,6 @@ extern "C" int jsd_nonlinear(jsd_plan* p, const void* vel_hat, const void* fwd_h
                        (double2*)h_g, p->fs);
     HIPCHK(hipGetLastError());
   }
-  // conv_ns = -H
-  const size_t n = 6 * p->fs;
-  hipLaunchKernelGGL(k_negate, dim3((n + 255) / 256), dim3(256), 0, st, (const double*)p->H, (double*)conv_ns, n);
-  HIPCHK(hipGetLastError());
   return JSD_OK;
 }
 
@@ -3162,7 +3192,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_ARG, "run %d: missing plan or buffer", r);
     }
-    if (!p->d.linearise || p->d.kind == JSD_KIND_NS || p->HGC) {
+    if (!p->d.linearise || p->d.kind == JSD_KIND_NS || p->HGC || p->dual_cc) {
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_STATE, "run %d: batches hold linearised perturbation or dual plans (coupling term on) only", r);
     }

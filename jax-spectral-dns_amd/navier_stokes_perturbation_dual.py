@@ -53,6 +53,9 @@ class NavierStokesVelVortPerturbationDual:
             raise ValueError("checkpoint interval must divide the number of steps")
         self.number_of_outer_steps = n // self.number_of_inner_steps
         fplan = fwd.plan()
+        # coupling_term = False: h_v gains (d_x^2 + d_z^2) hat(U' v_x)
+        # (navier_stokes_perturbation_dual.py:269-278)
+        self.coupling_term = bool(params.get("coupling_term", True))
         self._plan = StepPlan(
             fplan.grid,
             fplan.re_tau,
@@ -63,6 +66,7 @@ class NavierStokesVelVortPerturbationDual:
             ops=fplan.ops,
             explicit_dt=self.dt,
             plan_re=self.Re_tau,
+            coupling_term=self.coupling_term,
         )
         self.velocity_u_hat_0: Optional[torch.Tensor] = None
         self.u_T: Optional[torch.Tensor] = None
@@ -83,7 +87,6 @@ class NavierStokesVelVortPerturbationDual:
     # forward (navier_stokes_perturbation_dual.py:850-939)
     def run_forward_calculation(self) -> None:
         fwd = self.forward_equation
-        fplan = fwd.plan()
         u = fwd._state(fwd.get_initial_field("velocity_hat").get_data()).clone()
         self.velocity_u_hat_0 = u.clone()
         n, inner = self.number_of_time_steps, self.number_of_inner_steps
@@ -91,13 +94,13 @@ class NavierStokesVelVortPerturbationDual:
         self._history, self._checkpoints = [], []
         if need_history and not self.checkpointing:
             self._history.append(u.clone())
-            for _ in range(n):
-                fplan.step(u, 0.0, 1)
+            for k in range(n):
+                self._advance_forward(u, k, 1)
                 self._history.append(u.clone())
         else:
             self._checkpoints.append(u.clone())
-            for _ in range(self.number_of_outer_steps):
-                fplan.step(u, 0.0, inner)
+            for o in range(self.number_of_outer_steps):
+                self._advance_forward(u, o * inner, inner)
                 if need_history:
                     self._checkpoints.append(u.clone())
         self.u_T = u
@@ -107,6 +110,14 @@ class NavierStokesVelVortPerturbationDual:
             # checkpoints (navier_stokes_perturbation_dual.py:877-889)
             states = self._history or (self._checkpoints if len(self._checkpoints) > 1 else [self.velocity_u_hat_0, u])
             write_trajectory(torch.stack(states))
+
+    def _advance_forward(self, u: torch.Tensor, k0: int, nsteps: int) -> None:
+        """Forward steps k0 .. k0 + nsteps - 1 in place."""
+        self.forward_equation.plan().step(u, 0.0, nsteps)
+
+    def _dual_step(self, v: torch.Tensor, t: int, fwd_state: Optional[torch.Tensor]) -> None:
+        """Dual step t (0: from the final time) in place."""
+        self._plan.dual_step(v, fwd_state)
 
     def _forward_state(self, t: int, chunk: List[torch.Tensor], start: int) -> Optional[torch.Tensor]:
         if self.linearise_flag:
@@ -120,21 +131,21 @@ class NavierStokesVelVortPerturbationDual:
         if self.u_T is None:
             self.run_forward_calculation()
         assert self.u_T is not None
-        fplan = self.forward_equation.plan()
         v = -self.u_T.clone()  # adjoint initial condition for the gain (dual.py:904-907)
         n, inner = self.number_of_time_steps, self.number_of_inner_steps
         if self.linearise_flag or not self.checkpointing:
             for t in range(n):
-                self._plan.dual_step(v, self._forward_state(t, [], 0))
+                self._dual_step(v, t, self._forward_state(t, [], 0))
         else:
             for o in range(self.number_of_outer_steps):
                 u = self._checkpoints[-1 - (o + 1)].clone()
+                k0 = n - (o + 1) * inner  # forward step index of the chunk's start
                 chunk = [u.clone()]
-                for _ in range(inner):
-                    fplan.step(u, 0.0, 1)
+                for j in range(inner):
+                    self._advance_forward(u, k0 + j, 1)
                     chunk.append(u.clone())
                 for t in range(o * inner, (o + 1) * inner):
-                    self._plan.dual_step(v, self._forward_state(t, chunk, o * inner))
+                    self._dual_step(v, t, self._forward_state(t, chunk, o * inner))
         self.v_0 = v
         return v
 

@@ -96,7 +96,9 @@ class StepPlan:
         # eigen form of the implicit solve (shared E, E^-1 per parity half
         # instead of per-|k| inverse tables); False keeps the tables
         desc.y_eig = _ptr(arr(o.y_eig)) if eigen_solve else None
-        # coupling_term = False adds d_z hat(U' v) to h_g (navier_stokes_perturbation.py:214-224)
+        # coupling_term = False: perturbation plans add d_z hat(U' v) to h_g
+        # (navier_stokes_perturbation.py:214-224), dual plans (d_x^2 + d_z^2) hat(U' v_x)
+        # to h_v (navier_stokes_perturbation_dual.py:269-278)
         desc.no_coupling_term = 0 if coupling_term else 1
         self.has_adjoint = bool(adjoint) or (kind == _lib.KIND_PERTURBATION and self.linearise)
         h = ctypes.c_void_p()

@@ -755,6 +755,38 @@ def make_perturbation_instationary(
     return sp
 
 
+def make_dual_instationary(
+    dom: Dom,
+    re: float,
+    dt: float,
+    n_steps: int,
+    history: Callable[[int], torch.Tensor],
+    kappa: float = 1.0,
+    accelerate: bool = True,
+    n_max: int = 100,
+) -> StepParams:
+    """NavierStokesVelVortPerturbationInstationaryDual
+    (navier_stokes_perturbation_instationary_dual.py:70-119): the nonlinear
+    dual step (linearise = False, coupling_term = False: h_v gains
+    (d_x^2 + d_z^2) hat(U' v_x), navier_stokes_perturbation_dual.py:269-278)
+    of dual time step t about the forward base flow at time step n - t
+    (vel_base_fn, :96-101) and the forward state history(t) = u at step n - t
+    (navier_stokes_perturbation_dual.py:591-602); the forward's operators,
+    explicit terms with -dt, no pressure gradient (:83-89)."""
+    ops = build_operators(dom, re, dt)
+    sp = StepParams(dom, ops, -dt, 0.0)
+    X0, X1, X2 = np.meshgrid(dom.x, dom.y, dom.z, indexing="ij")
+    zero = field_hat(dom, torch.from_numpy(0.0 * X0 * X1 * X2))
+
+    def base(k: int) -> torch.Tensor:
+        U = instationary_base_profile(dom.y, k * dt, re, kappa, accelerate, n_max)
+        ub = torch.from_numpy(U[None, :, None] + 0.0 * X0 * X2)
+        return torch.stack([field_hat(dom, ub), zero, zero])
+
+    sp.nonlinear = lambda vh, t: nl_dual_skew(dom, vh, base(n_steps - t), history(t), False, coupling_term=False)
+    return sp
+
+
 def make_ns(dom: Dom, re: float, dt: float, dPdx: float, u_max_over_u_tau: float = 1.0) -> StepParams:
     re_tau = re / u_max_over_u_tau
     ops = build_operators(dom, re_tau, dt)

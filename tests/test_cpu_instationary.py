@@ -45,3 +45,33 @@ def test_profile_matches_oracle(accelerate):
         a = instationary_base_profile(y, t, 800.0, 2.0, accelerate, 100)
         b = O.instationary_base_profile(y, t, 800.0, 2.0, accelerate, 100)
         assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+
+
+def test_dual_coupling_correction_closed_form():
+    """Dual term with coupling_term = False (navier_stokes_perturbation_dual.py:269-278)
+    about U = 1 - y^2: it differs from the coupled term only in h_v, by
+    (d_x^2 + d_z^2) hat(U' v_x) with U' = -2y taken analytically, v_x the
+    x-component of the dual field.  A correction on the wrong component or
+    term, with the wrong sign or with d_z in place of d_x^2 + d_z^2 fails."""
+    import torch
+
+    from tests.helpers import perturbation_field
+
+    d = O.make_domain((8, 17, 8), 2 * np.pi, np.pi)
+    g = DM.make_grid((8, 17, 8), 2 * np.pi, np.pi)
+    base = O.base_flow_hat(d)
+    v = perturbation_field(g, seed=61, energy=1e-2)
+    u = perturbation_field(g, seed=62, energy=1e-2)
+    a = O.nl_dual_skew(d, v, base, u, False, coupling_term=True)
+    b = O.nl_dual_skew(d, v, base, u, False, coupling_term=False)
+    for k in (1, 2, 3):
+        assert torch.equal(a[k], b[k])
+    vx = O.field_no_hat(d, v[0])
+    y = torch.from_numpy(np.asarray(d.y))[None, :, None]
+    corr = O.field_hat(d, -2.0 * y * vx)
+    kx = torch.from_numpy(np.asarray(d.kx))[:, None, None]
+    kz = torch.from_numpy(np.asarray(d.kz))[None, None, :]
+    expect = -(kx**2 + kz**2) * corr
+    diff = b[0] - a[0]
+    assert float((diff - expect).abs().max()) < 1e-11 * float(expect.abs().max())
+    assert float(expect.abs().max()) > 1e-6 * float(a[0].abs().max())
