@@ -34,7 +34,7 @@ FP64_MFMA_PEAK_TFS = 78.6
 TRAFFIC_FILE = "hbm_traffic_ns_128x129x128.json"
 CLASSES = ("ygemm", "fft_x_inv", "fft_z_mid", "fft_x_fwd", "ygemm_hv", "solve", "nl_lin")
 KERNEL_NAMES = {
-    "ygemm": "k_ypar",
+    "ygemm": "k_ypar_stream<true, true>",
     "fft_x_inv": "k_fft_x_inv_t<Rad<8, 5, 5, 1> >",
     "fft_z_mid": "k_fft_z_mid_256<6, 0>",
     "fft_x_fwd": "k_fft_x_fwd_t<Rad<8, 5, 5, 1> >",
