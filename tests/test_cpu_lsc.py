@@ -103,3 +103,17 @@ def test_single_mode_field_and_energy_over_time():
     assert abs(fn(0.0) - 0.01 * e0) < 1e-14 * e0
     assert abs(fn(2.0) / fn(0.0) - np.exp(4.0 * lam.real)) < 1e-14
     assert abs(float(O.energy(d, hat)) - e0) < 1e-12 * e0
+
+
+def test_single_mode_fields_saved_as_field_files(tmp_path, monkeypatch):
+    """save=True writes the three components as field files
+    (linear_stability_calculation.py:262-265) that FromFile reads back."""
+    from jax_spectral_dns_amd.field import Field, PhysicalField
+
+    monkeypatch.setattr(Field, "field_dir", str(tmp_path))
+    dom = PhysicalDomain.create((4, 33, 4), (True, False, True), scale_factors=(2 * np.pi, 1.0, 1e-3))
+    lsc = LinearStabilityCalculation(Re=6000.0, alpha=1.0, beta=0.0, n=33)
+    u = lsc.velocity_field_single_mode(dom, save=True)
+    for c, comp in zip("uvw", u):
+        back = PhysicalField.FromFile(dom, "%s_Re_%s_n_%d_mode_%d" % (c, lsc.Re, lsc.n, 0), name=comp.name)
+        assert torch.equal(back.get_data(), comp.get_data())
