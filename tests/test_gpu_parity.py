@@ -33,10 +33,12 @@ def test_ydiff(shape, lx, lz):
         assert rel(out, ref) < YDIFF_TOL[order], order
 
 
-@pytest.mark.parametrize("shape", [(16, 48, 16), (8, 130, 8)])
+@pytest.mark.parametrize("shape", [(16, 48, 16), (8, 130, 8), (8, 24, 8), (8, 127, 8), (8, 128, 8)])
 def test_ydiff_even_and_largest_ny(shape):
-    """Even Ny (no centre point in the even/odd split) and the largest Ny the
-    parity blocks hold."""
+    """Even Ny (no centre point in the even/odd split), the largest Ny the
+    parity blocks hold, Ny = 24 (fewer k-steps than the streaming kernel's
+    prefetch distance: the blocked kernel runs), and Ny = 127 / 128 (every
+    MFMA tile row stored; 127 has its centre row inside the tiles)."""
     g, d = pair(shape)
     plan = StepPlan(g, 3000.0, 0.01, KIND_NS)
     rng = np.random.default_rng(11)
