@@ -385,7 +385,10 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
   const int MC2 = G.M * C2;
   const int nunit = (ncol + 15) / 16;
   const int W = gridDim.x * (YS_THREADS / 64);
-  const int u0 = blockIdx.x * (YS_THREADS / 64) + wave;
+  // wave-major numbering: the units past the first W go to waves 0, 1, 2, ...
+  // of every workgroup in turn, so they spread over the SIMDs of all CUs
+  // instead of doubling the work of the first workgroups' CUs
+  const int u0 = wave * gridDim.x + blockIdx.x;
   const double* __restrict__ in = G.in;
   double* __restrict__ out = G.out;
   const double* __restrict__ sC = sF + (size_t)KL * 8 * 64 + kr;
@@ -399,15 +402,16 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
   };
   // rows 4 ks + kr and N-1-(4 ks + kr).  Branch-free: a lane outside the
   // data reads a zero word instead, so every position issues the same two
-  // loads.  dep (the value just taken from the slot being refilled) enters
-  // the address through an empty asm, so the refill cannot be scheduled
-  // before that use and reuses the slot's registers: no copies of in-flight
-  // values at the loop edge, and the waits before the MFMAs leave the newer
-  // prefetches in flight.
-  auto fetch = [&](int base, int ks, double dep, double& xa, double& xb) {
+  // loads.  dep, dep2 (e and o, the two values computed from the slot being
+  // refilled) enter the addresses through an empty asm, so the refill cannot
+  // be scheduled before the slot's last use and takes the slot's registers:
+  // no copies of in-flight values at the loop edge (which would wait for the
+  // newest loads), and the waits before the MFMAs leave the newer prefetches
+  // in flight.
+  auto fetch = [&](int base, int ks, double dep, double dep2, double& xa, double& xb) {
     const bool v = base >= 0 && 4 * ks + kr < nEf;
     int oa = base + kr * C2 + 4 * ks * C2, ob = base + (N - 1 - kr) * C2 - 4 * ks * C2;
-    __asm__("" : "+v"(oa), "+v"(ob) : "v"(dep));
+    __asm__("" : "+v"(oa), "+v"(ob) : "v"(dep), "v"(dep2));
     xa = *(v ? in + oa : G.zero);
     xb = *(v ? in + ob : G.zero);
   };
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
   double ra[YS_PD], rb[YS_PD];
 #pragma unroll
   for (int j = 0; j < YS_PD; ++j) {
-    fetch(fb, fk, 0.0, ra[j], rb[j]);
+    fetch(fb, fk, 0.0, 0.0, ra[j], rb[j]);
     if (++fk == KL) {
       fk = 0;
       fu += W;
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
 #pragma unroll
     for (int j = 0; j < YS_PD; ++j) {
       const double e = 0.5 * (ra[j] + rb[j]), o = 0.5 * (ra[j] - rb[j]);
-      fetch(fb, fk, o, ra[j], rb[j]);
+      fetch(fb, fk, e, o, ra[j], rb[j]);
       if (++fk == KL) {
         fk = 0;
         fu += W;
