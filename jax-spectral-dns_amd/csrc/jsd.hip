@@ -385,10 +385,18 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
   const int MC2 = G.M * C2;
   const int nunit = (ncol + 15) / 16;
   const int W = gridDim.x * (YS_THREADS / 64);
-  // wave-major numbering: the units past the first W go to waves 0, 1, 2, ...
-  // of every workgroup in turn, so they spread over the SIMDs of all CUs
-  // instead of doubling the work of the first workgroups' CUs
-  const int u0 = wave * gridDim.x + blockIdx.x;
+  // Unit of round r of this wave.  Round 0 is block-major (a workgroup's
+  // eight waves take eight neighbouring units, whose rows share 128-byte
+  // lines, in one XCD's L2).  Later rounds hand out chunks of four
+  // neighbouring units to waves 0-3 of every workgroup in turn, then to
+  // waves 4-7, so a partial last round spreads over the SIMDs of all CUs
+  // instead of doubling the work of the first workgroups' CUs.
+  const int blk = blockIdx.x, G8 = gridDim.x;
+  auto unit_of = [&](int r) -> int {
+    if (r == 0) return blk * 8 + wave;
+    return r * W + 4 * (G8 * (wave >> 2) + blk) + (wave & 3);
+  };
+  const int u0 = unit_of(0);
   const double* __restrict__ in = G.in;
   double* __restrict__ out = G.out;
   const double* __restrict__ sC = sF + (size_t)KL * 8 * 64 + kr;
@@ -415,21 +423,21 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
     xa = *(v ? in + oa : G.zero);
     xb = *(v ? in + ob : G.zero);
   };
-  const int nu = u0 < nunit ? (nunit - u0 + W - 1) / W : 0;
+  const int nfull = nunit / W;
+  const int nu = nfull + (unit_of(nfull) < nunit ? 1 : 0);
   const int npos = ((nu * KL + YS_PD - 1) / YS_PD) * YS_PD;
   // prefetch cursor (unit base, k-step) and compute cursor
-  int fu = u0, fb = col_base(u0), fk = 0;
+  int fr = 0, fb = col_base(u0), fk = 0;
   double ra[YS_PD], rb[YS_PD];
 #pragma unroll
   for (int j = 0; j < YS_PD; ++j) {
     fetch(fb, fk, 0.0, 0.0, ra[j], rb[j]);
     if (++fk == KL) {
       fk = 0;
-      fu += W;
-      fb = col_base(fu);
+      fb = col_base(unit_of(++fr));
     }
   }
-  int cu = u0, cb = col_base(u0), ck = 0;
+  int cr = 0, cb = col_base(u0), ck = 0;
   dbl4 ae[4], ao[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) ae[t] = ao[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
@@ -441,8 +449,7 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
       fetch(fb, fk, e, o, ra[j], rb[j]);
       if (++fk == KL) {
         fk = 0;
-        fu += W;
-        fb = col_base(fu);
+        fb = col_base(unit_of(++fr));
       }
       const double inE = ANTI ? o : e, inO = ANTI ? e : o;
       const double* __restrict__ f = sF + (size_t)ck * 512 + lane;
@@ -486,8 +493,7 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
         for (int t = 0; t < 4; ++t) ae[t] = ao[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
         pe = po = 0.0;
         ck = 0;
-        cu += W;
-        cb = col_base(cu);
+        cb = col_base(unit_of(++cr));
       }
     }
   }
