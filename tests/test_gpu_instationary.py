@@ -145,3 +145,25 @@ def test_instationary_dual_run(checkpoint_every):
     err = rel(v0, v)
     print("instationary dual rel %.3e" % err)
     assert err < 1e-10
+
+
+def test_coupling_off_rejections():
+    """coupling_term = False is refused where it has no meaning or no
+    implementation: on NS plans, in batches, and in the discrete adjoint.
+    All three are host-side checks made before any kernel runs."""
+    from jax_spectral_dns_amd import KIND_DUAL, KIND_NS
+    from jax_spectral_dns_amd._lib import JsdError
+    from jax_spectral_dns_amd.batch import BatchPlan
+
+    g, _ = pair((24, 33, 16), 2 * np.pi, np.pi)
+    with pytest.raises(JsdError):
+        StepPlan(g, 500.0, 0.01, KIND_NS, coupling_term=False)
+    fwd = StepPlan(g, 500.0, 0.01, KIND_PERTURBATION, linearise=True)
+    dual = StepPlan(g, 500.0, 0.01, KIND_DUAL, linearise=True, ops=fwd.ops, explicit_dt=-0.01, plan_re=-500.0,
+                    coupling_term=False)
+    state = torch.zeros(dual.shape, dtype=torch.complex128, device=DEV)
+    with pytest.raises(JsdError):
+        BatchPlan([dual], [state])
+    pert = StepPlan(g, 500.0, 0.01, KIND_PERTURBATION, linearise=True, coupling_term=False)
+    with pytest.raises(JsdError):
+        pert.vjp(torch.zeros(pert.shape, dtype=torch.complex128, device=DEV), 1)
