@@ -19,11 +19,13 @@ from __future__ import annotations
 
 from typing import Any, List, Optional
 
+import numpy as np
 import torch
 
 from .field import FourierField, VectorField
 from .navier_stokes_perturbation import NavierStokesVelVortPerturbation
 from .navier_stokes_perturbation_dual import NavierStokesVelVortPerturbationDual
+from .plan import StepPlan
 
 
 class SteepestAdaptiveDescentSolver:
@@ -202,4 +204,205 @@ class ConjugateGradientDescentSolver:
             self.update()
             if self.is_done():
                 break
+        return self.values
+
+
+def _rdot(a: torch.Tensor, b: torch.Tensor) -> float:
+    """Real inner product of two spectra viewed as real vectors."""
+    return float(torch.sum(a.real * b.real + a.imag * b.imag))
+
+
+class OptimiserWrapper:
+    """Restates OptimiserWrapper (gradient_descent_solver.py:862-934), the
+    driver that hands the gain and its dual gradient to an external
+    optimiser: jaxopt.NonlinearCG with a zoom line search (optimiser.py:231-242,
+    use_optax = False).  jaxopt is not available here, so the same method is
+    written out from Nocedal & Wright, Numerical Optimization (2006):
+
+    * the objective is the reference's, f = -gain (run_adjoint,
+      minimise = False).  Its gradient is the exact one: the discrete adjoint
+      of the step chained with the reference's gradient parameterisation
+      (optimiser.gain_and_grad_parameters; parameters omega_y, v, u_00,
+      w_00 through vort_yvel_to_vel, navier_stokes.py:1063-1203).  The
+      parameterisation keeps every iterate solenoidal and no-slip, and the
+      gain is scale-invariant in the parameters.  The reference's wrapper
+      passes the velocity spectrum itself as parameters, with
+      -get_grad = -(u0 - v0)/E0 of the continuous-adjoint (dual) solution.
+      Two things go wrong when that pair is used as written:
+        - that vector is not the Euclidean gradient a Wolfe line search
+          needs (it is an ascent direction in the energy metric), so no step
+          along it satisfies sufficient decrease;
+        - the exact gradient over the raw spectrum leaves the solenoidal
+          subspace (the step sees only v and omega_y) and inflates the gain
+          without bound;
+    * nonlinear conjugate gradients, Polak-Ribiere with beta clipped at 0
+      (PR+), restarting along -g when the direction is not a descent
+      direction;
+    * the step along d satisfies the strong Wolfe conditions (c1 = 1e-4,
+      c2 = 0.1, the curvature constant Nocedal & Wright give for nonlinear
+      CG) by bracketing and zoom (Algorithms 3.5 and 3.6) with
+      safeguarded cubic interpolation, at most maxls = 15 evaluations,
+      starting from the previous accepted step ("current" initialisation);
+      if no point in the budget satisfies both conditions, the lowest
+      sufficient-decrease point is taken, and none at all means no move and
+      a restart along -g.
+
+    After each iteration current_guess is N(x) and value its gain
+    (update, :936-944)."""
+
+    c1 = 1e-4
+    c2 = 0.1
+
+    def __init__(self, nse: NavierStokesVelVortPerturbation, **params: Any):
+        self.nse = nse
+        self.number_of_steps = int(params.get("max_iterations", params.get("max_iter", 20)))
+        self.maxls = int(params.get("maxls", 15))
+        self.step_size = float(params.get("step_size", 1e-2))
+        self.re = nse.get_Re_tau() * nse.get_u_max_over_u_tau()
+        self.dt = nse.get_dt()
+        self.end_time = nse.end_time
+        self.linearise = nse.linearise()
+        self.physical_domain = nse.get_physical_domain()
+        u_init = nse.get_initial_field("velocity_hat").get_data()
+        fplan = nse.plan()
+        # the discrete adjoint: linearised perturbation plans always carry it,
+        # nonlinear ones are rebuilt with adjoint = True
+        self.plan = fplan if self.linearise else StepPlan(
+            fplan.grid, fplan.re_tau, fplan.dt, fplan.kind, False, nse.u_max_over_u_tau, ops=fplan.ops, adjoint=True
+        )
+        self.nsteps = nse.number_of_time_steps()
+        self.checkpoint_every = int(params.get("checkpoint_every", 50))
+        self.e_0 = self.plan.energy(u_init)
+        g = self.plan.grid
+        dev = u_init.device
+        kx = torch.as_tensor(g.kx, device=dev)[:, None, None]
+        kz = torch.as_tensor(g.kz, device=dev)[None, None, :]
+        # (omega_y, v, u_00, w_00) of the initial field
+        init = [1j * kz * u_init[0] - 1j * kx * u_init[2], u_init[1], u_init[0, 0, :, 0], u_init[2, 0, :, 0]]
+        self._shapes = [t.shape for t in init]
+        self.x = self._pack(init)
+        from .parameterisation import VelocityParameterisation
+
+        self._param = VelocityParameterisation(g, dev)
+        self.evaluations = 0
+        self.f, self.g = self.value_and_grad(self.x)
+        self.d = -self.g
+        self.current_guess = self.velocity(self.x)
+        self.value = -self.f
+        self.values: List[float] = [self.value]
+        self.steps: List[float] = []
+        self.wolfe: List[bool] = []
+        self.i = 0
+
+    def _pack(self, ts) -> torch.Tensor:
+        return torch.cat([t.reshape(-1) for t in ts])
+
+    def _unpack(self, x: torch.Tensor):
+        out, o = [], 0
+        for sh in self._shapes:
+            n = int(np.prod(sh))
+            out.append(x[o : o + n].reshape(sh))
+            o += n
+        return out
+
+    def normalize_field(self, v: torch.Tensor) -> torch.Tensor:
+        return v * (self.e_0 / self.plan.energy(v)) ** 0.5
+
+    def velocity(self, x: torch.Tensor) -> torch.Tensor:
+        """The initial velocity spectrum of parameters x at the initial energy
+        (parameters_to_run_input and normalize_current_guess)."""
+        with torch.no_grad():
+            return self.normalize_field(self._param(*self._unpack(x)))
+
+    def value_and_grad(self, x: torch.Tensor):
+        """run_adjoint (gradient_descent_solver.py:880-906): -gain of the
+        parameters' initial field and its exact gradient."""
+        from .optimiser import gain_and_grad_parameters
+
+        gain, grads = gain_and_grad_parameters(self.plan, self._unpack(x), self.nsteps, self.checkpoint_every)
+        self.evaluations += 1
+        return -gain, -self._pack(grads)
+
+    def _line_search(self, d: torch.Tensor, dphi0: float):
+        """Strong Wolfe step along d from (self.x, self.f); returns
+        (alpha, f, g, wolfe) or None."""
+        f0 = self.f
+        evals = [0]
+        best = [None]
+
+        def phi(a: float):
+            f, g = self.value_and_grad(self.x + a * d)
+            evals[0] += 1
+            if f <= f0 + self.c1 * a * dphi0 and (best[0] is None or f < best[0][1]):
+                best[0] = (a, f, g, False)
+            return f, g, _rdot(g, d)
+
+        def zoom(lo, f_lo, dp_lo, hi, f_hi, dp_hi):
+            while evals[0] < self.maxls:
+                # cubic interpolation of phi on [lo, hi], kept inside the
+                # middle 80 % of the interval, else bisection
+                d1 = dp_lo + dp_hi - 3.0 * (f_lo - f_hi) / (lo - hi)
+                rad = d1 * d1 - dp_lo * dp_hi
+                a = 0.5 * (lo + hi)
+                if rad >= 0.0:
+                    d2 = (1.0 if hi > lo else -1.0) * rad**0.5
+                    den = dp_hi - dp_lo + 2.0 * d2
+                    if den != 0.0:
+                        c = hi - (hi - lo) * (dp_hi + d2 - d1) / den
+                        lo_b, hi_b = min(lo, hi), max(lo, hi)
+                        w = hi_b - lo_b
+                        if lo_b + 0.1 * w <= c <= hi_b - 0.1 * w:
+                            a = c
+                f, g, dp = phi(a)
+                if f > f0 + self.c1 * a * dphi0 or f >= f_lo:
+                    hi, f_hi, dp_hi = a, f, dp
+                else:
+                    if abs(dp) <= -self.c2 * dphi0:
+                        return a, f, g, True
+                    if dp * (hi - lo) >= 0.0:
+                        hi, f_hi, dp_hi = lo, f_lo, dp_lo
+                    lo, f_lo, dp_lo = a, f, dp
+            return best[0]
+
+        a_prev, f_prev, dp_prev = 0.0, f0, dphi0
+        a = self.step_size
+        while evals[0] < self.maxls:
+            f, g, dp = phi(a)
+            if f > f0 + self.c1 * a * dphi0 or (a_prev > 0.0 and f >= f_prev):
+                return zoom(a_prev, f_prev, dp_prev, a, f, dp)
+            if abs(dp) <= -self.c2 * dphi0:
+                return a, f, g, True
+            if dp >= 0.0:
+                return zoom(a, f, dp, a_prev, f_prev, dp_prev)
+            a_prev, f_prev, dp_prev = a, f, dp
+            a *= 2.0
+        return best[0]
+
+    def update(self) -> None:
+        d = self.d
+        dphi0 = _rdot(self.g, d)
+        if dphi0 >= 0.0:  # not a descent direction: restart
+            d = -self.g
+            dphi0 = _rdot(self.g, d)
+        res = self._line_search(d, dphi0)
+        if res is None:
+            self.d = -self.g
+            self.wolfe.append(False)
+        else:
+            a, f, g, wolfe = res
+            beta = max(_rdot(g, g - self.g) / _rdot(self.g, self.g), 0.0)
+            self.x = self.x + a * d
+            self.f, self.step_size = f, a
+            self.d = -g + beta * d
+            self.g = g
+            self.steps.append(a)
+            self.wolfe.append(wolfe)
+        self.current_guess = self.velocity(self.x)
+        self.value = -self.f
+        self.values.append(self.value)
+        self.i += 1
+
+    def optimise(self, iterations: Optional[int] = None) -> List[float]:
+        for _ in range(self.number_of_steps if iterations is None else iterations):
+            self.update()
         return self.values
