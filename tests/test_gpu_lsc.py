@@ -67,7 +67,7 @@ def test_oblique_eigenmodes_decay_at_the_linear_theory_rate(alpha, beta):
     the Orr-Sommerfeld / Squire problem for an oblique (alpha, beta) wave at
     Re 1000, placed on a box where that wave is an interior stored mode,
     decay under the linearised GPU step at 2 Re(lambda) in energy
-    (t = 1, dt = 0.01), Squire modes included."""
+    (t = 1, dt = 0.01, to 1e-4 relative), Squire modes included."""
     import torch
 
     re, dt, T = 1000.0, 0.01, 1.0
@@ -84,4 +84,6 @@ def test_oblique_eigenmodes_decay_at_the_linear_theory_rate(alpha, beta):
         plan.step(hat, 0.0, int(round(T / dt)))
         rate = np.log(plan.energy(hat) / e0) / (2 * T)
         print((alpha, beta), mode, "lambda %.6f, DNS %.6f" % (lam[mode].real, rate))
-        assert abs(rate - lam[mode].real) < 1e-5 * abs(lam[mode].real) + 1e-8
+        # RK3/CN at dt = 0.01 leaves ~1e-5 relative in the decay of the
+        # faster waves (|Im lambda| ~ 2); a wrong operator errs at O(1)
+        assert abs(rate - lam[mode].real) < 1e-4 * abs(lam[mode].real) + 1e-8
