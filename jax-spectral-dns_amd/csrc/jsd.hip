@@ -966,40 +966,49 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
 // t and k2) takes the 16-point DFT over n1, giving the physical point
 // z = 16 k1 + k2.  Two LDS exchanges per transform instead of three stages,
 // no LDS zero fill.  The forward transform of the two packed product rows
-// runs the same two steps and stores only the kept bins.
+// runs the same two steps and stores only the kept bins.  A workgroup takes
+// LN neighbouring y lines of one x: with LN = 2 and 6 rows the butterfly
+// steps fill 3 of 4 waves and the forward steps one whole wave (one line
+// leaves half of a wave idle in both).
 #define Z16_LD 17  // padded row (in double2) of the [t][n1][k2] exchange layout
 
-template <int NT>
-__host__ __device__ constexpr int z256_threads() { return NT * 16 > 128 ? NT * 16 : 128; }
+template <int NT, int LN = 1>
+__host__ __device__ constexpr int z256_threads() {
+  return NT * 16 * LN > 128 ? (LN == 1 ? NT * 16 : 256) : 128;
+}
 
 // PZ: at most 129 stored kz (|kz| <= 64), so step 1 of the inverse sees
 // zeros at n2 = 5..11 (dft16_pz).
-template <int NT, int PROD, bool PZ>
-__global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
+template <int NT, int PROD, bool PZ, int LN = 1>
+__global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
     const double2* __restrict__ T, long tstride, double2* __restrict__ V, long vstride,
     const double2* __restrict__ tw, int N, int Z, int ZT, double clin, const double* __restrict__ Ub,
     const double* __restrict__ dUb) {
   constexpr int P = 256;
-  constexpr int NTH = z256_threads<NT>();
-  constexpr int ITEMS = (P + NTH - 1) / NTH;
-  // [t][n1][k2] (padded) for the exchanges, [t][z] for the products
-  __shared__ double2 lds[NT * 16 * Z16_LD > P * NT ? NT * 16 * Z16_LD : P * NT];
+  constexpr int NTH = z256_threads<NT, LN>();
+  static_assert(NT * 16 * LN <= NTH && 32 * LN <= NTH, "rows of the workgroup exceed its threads");
+  constexpr int ITEMS = (LN * P + NTH - 1) / NTH;
+  // per line: [t][n1][k2] (padded) for the exchanges, [t][z] for the products
+  constexpr int REG = NT * 16 * Z16_LD > P * NT ? NT * 16 * Z16_LD : P * NT;
+  __shared__ double2 lds[LN * REG];
   const int x = blockIdx.x;
-  const int l = blockIdx.y;
+  const int l0 = blockIdx.y * LN;  // lines l0 .. l0+LN-1; those >= N load zeros and store nothing
   const int KZ = Z / 2;
   const int nneg = Z - KZ - 1;  // stored negative bins
-  const size_t line = ((size_t)x * N + l) * ZT;
   double2 v[16];
   // ---- inverse, step 1
-  if (threadIdx.x < NT * 16) {
-    const int t = threadIdx.x >> 4, n1 = threadIdx.x & 15;
+  if (threadIdx.x < LN * NT * 16) {
+    const int li = threadIdx.x / (NT * 16), r = threadIdx.x % (NT * 16);
+    const int t = r >> 4, n1 = r & 15;
+    const int l = l0 + li;
+    const size_t line = ((size_t)x * N + (l < N ? l : 0)) * ZT;
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
       const int n = n1 + 16 * n2;
       int j = -1;
       if (n <= KZ) j = n;
       else if (n >= P - nneg) j = n - P + Z;
-      v[n2] = j >= 0 ? ld_nt(&T[t * tstride + line + j]) : make_double2(0.0, 0.0);
+      v[n2] = (j >= 0 && l < N) ? ld_nt(&T[t * tstride + line + j]) : make_double2(0.0, 0.0);
     }
     if constexpr (PZ) dft16_pz<+1>(v);
     else dft16<+1>(v);
@@ -1007,75 +1016,84 @@ __global__ __launch_bounds__(z256_threads<NT>()) void k_fft_z_mid_256(
     for (int k2 = 0; k2 < 16; ++k2) {
       double2 y = v[k2];
       if (n1 * k2 > 0) y = cmul(y, cconj(tw[n1 * k2]));
-      lds[(t * 16 + n1) * Z16_LD + k2] = y;
+      lds[li * REG + (t * 16 + n1) * Z16_LD + k2] = y;
     }
   }
   __syncthreads();
   // ---- inverse, step 2
-  if (threadIdx.x < NT * 16) {
-    const int t = threadIdx.x >> 4, k2 = threadIdx.x & 15;
+  if (threadIdx.x < LN * NT * 16) {
+    const int li = threadIdx.x / (NT * 16), r = threadIdx.x % (NT * 16);
+    const int t = r >> 4, k2 = r & 15;
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[(t * 16 + n1) * Z16_LD + k2];
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[li * REG + (t * 16 + n1) * Z16_LD + k2];
     dft16<+1>(v);
   }
   __syncthreads();
-  if (threadIdx.x < NT * 16) {
-    const int t = threadIdx.x >> 4, k2 = threadIdx.x & 15;
+  if (threadIdx.x < LN * NT * 16) {
+    const int li = threadIdx.x / (NT * 16), r = threadIdx.x % (NT * 16);
+    const int t = r >> 4, k2 = r & 15;
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) lds[t * P + 16 * k1 + k2] = v[k1];
+    for (int k1 = 0; k1 < 16; ++k1) lds[li * REG + t * P + 16 * k1 + k2] = v[k1];
   }
   __syncthreads();
   // ---- products at each physical point
-  const double U = Ub ? Ub[l] : 0.0;
-  const double dU = dUb ? dUb[l] : 0.0;
   double Hs[ITEMS][3];
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q) {
-    const int z = threadIdx.x + q * NTH;
-    if (z < P) zmid_products<PROD, P>(lds + z, U, dU, clin, Hs[q]);
+    const int zz = threadIdx.x + q * NTH;
+    const int li = zz / P, z = zz % P;
+    const int l = l0 + li < N ? l0 + li : N - 1;
+    const double U = Ub ? Ub[l] : 0.0;
+    const double dU = dUb ? dUb[l] : 0.0;
+    if (zz < LN * P) zmid_products<PROD, P>(lds + li * REG + z, U, dU, clin, Hs[q]);
   }
   __syncthreads();
   // packed rows (H0 + i H1) and (H2 + 0 i) at [o][z]
 #pragma unroll
   for (int q = 0; q < ITEMS; ++q) {
-    const int z = threadIdx.x + q * NTH;
-    if (z < P) {
-      lds[z] = make_double2(Hs[q][0], Hs[q][1]);
-      lds[P + z] = make_double2(Hs[q][2], 0.0);
+    const int zz = threadIdx.x + q * NTH;
+    const int li = zz / P, z = zz % P;
+    if (zz < LN * P) {
+      lds[li * REG + z] = make_double2(Hs[q][0], Hs[q][1]);
+      lds[li * REG + P + z] = make_double2(Hs[q][2], 0.0);
     }
   }
   __syncthreads();
-  // ---- forward, step 1 (rows o = 0, 1)
-  if (threadIdx.x < 32) {
-    const int o = threadIdx.x >> 4, n1 = threadIdx.x & 15;
+  // ---- forward, step 1 (rows o = 0, 1 of each line)
+  if (threadIdx.x < 32 * LN) {
+    const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, n1 = threadIdx.x & 15;
 #pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) v[n2] = lds[o * P + n1 + 16 * n2];
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = lds[li * REG + o * P + n1 + 16 * n2];
     dft16<-1>(v);
   }
   __syncthreads();
-  if (threadIdx.x < 32) {
-    const int o = threadIdx.x >> 4, n1 = threadIdx.x & 15;
+  if (threadIdx.x < 32 * LN) {
+    const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, n1 = threadIdx.x & 15;
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
       double2 y = v[k2];
       if (n1 * k2 > 0) y = cmul(y, tw[n1 * k2]);
-      lds[(o * 16 + n1) * Z16_LD + k2] = y;
+      lds[li * REG + (o * 16 + n1) * Z16_LD + k2] = y;
     }
   }
   __syncthreads();
   // ---- forward, step 2: kept bins to V
-  if (threadIdx.x < 32) {
-    const int o = threadIdx.x >> 4, k2 = threadIdx.x & 15;
+  if (threadIdx.x < 32 * LN) {
+    const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, k2 = threadIdx.x & 15;
+    const int l = l0 + li;
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[(o * 16 + n1) * Z16_LD + k2];
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[li * REG + (o * 16 + n1) * Z16_LD + k2];
     dft16<-1>(v);
+    if (l < N) {
+      const size_t line = ((size_t)x * N + l) * ZT;
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) {
-      const int b = 16 * k1 + k2;
-      int j = -1;
-      if (b <= KZ) j = b;
-      else if (b >= P - nneg) j = b - P + Z;
-      if (j >= 0) st_nt(&V[o * vstride + line + j], v[k1]);
+      for (int k1 = 0; k1 < 16; ++k1) {
+        const int b = 16 * k1 + k2;
+        int j = -1;
+        if (b <= KZ) j = b;
+        else if (b >= P - nneg) j = b - P + Z;
+        if (j >= 0) st_nt(&V[o * vstride + line + j], v[k1]);
+      }
     }
   }
 }
@@ -2446,15 +2464,15 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   prof_begin(p, 2, st);
   const bool z_256 = ct && plan_is<RZ256>(p->fpz);
   if (z_256) {
-    const dim3 grid(d.ppx, N);
+    const dim3 grid(d.ppx, N), grid2(d.ppx, (N + 1) / 2);
     auto launch = [&](auto pz) {
       constexpr bool PZ = decltype(pz)::value;
       if (prod == PROD_NS)
-        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_NS, PZ>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts,
-                           p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_NS, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
+                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
       else if (prod == PROD_PERT)
-        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_PERT, PZ>), grid, dim3(z256_threads<6>()), 0, st, p->T, (long)ts,
-                           p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_PERT, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
+                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
       else
         hipLaunchKernelGGL((k_fft_z_mid_256<12, PROD_DUAL, PZ>), grid, dim3(z256_threads<12>()), 0, st, p->T,
                            (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
