@@ -1015,7 +1015,9 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
       double2 y = v[k2];
-      if (n1 * k2 > 0) y = cmul(y, cconj(tw[n1 * k2]));
+      // tw[0] = (1, -0): for n1 = 0 the product leaves finite values unchanged,
+      // and with no branch on n1 the twiddle loads issue together
+      if (k2 > 0) y = cmul(y, cconj(tw[n1 * k2]));
       lds[li * REG + (t * 16 + n1) * Z16_LD + k2] = y;
     }
   }
@@ -1072,7 +1074,7 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
       double2 y = v[k2];
-      if (n1 * k2 > 0) y = cmul(y, tw[n1 * k2]);
+      if (k2 > 0) y = cmul(y, tw[n1 * k2]);  // unconditional loads, as in the inverse
       lds[li * REG + (o * 16 + n1) * Z16_LD + k2] = y;
     }
   }
