@@ -176,7 +176,8 @@ int jsd_profile(jsd_plan* plan, int enable, double* ms, long* count);
  * each plan's layout), bound at creation; adjoints may be NULL (no VJP).
  * Creation runs the autotuner on the stream and leaves the states
  * unchanged.  A batch uses its plans' workspaces: do not step those plans
- * concurrently with the batch. */
+ * concurrently with the batch.  Each run needs its own plan and buffers: a
+ * plan, state or adjoint buffer that appears twice is rejected (JSD_ERR_ARG). */
 typedef struct jsd_batch jsd_batch;
 int jsd_batch_create(jsd_plan* const* plans, void* const* states, void* const* adjoints, int nrun, void* stream,
                      jsd_batch** out);

@@ -25,6 +25,19 @@ from .plan import StepPlan, _dev, _stream
 STAGES = ("ygemm", "nl_lin", "solve", "solve_T", "hv_T", "nl_lin_T", "acc")
 
 
+def check_distinct(plans: Sequence[object], states: Sequence[torch.Tensor],
+                   adjoints: Optional[Sequence[torch.Tensor]] = None) -> None:
+    """Every run of a batch writes its own plan's workspaces inside each stage
+    launch, so two runs on one plan (or one state / adjoint buffer) would race
+    on them: reject repeated plans and buffers."""
+    if len({id(p) for p in plans}) != len(plans):
+        raise _lib.JsdError("a plan appears twice in the batch: each run needs its own StepPlan")
+    bufs = list(states) + (list(adjoints) if adjoints is not None else [])
+    ptrs = [b.data_ptr() for b in bufs]
+    if len(set(ptrs)) != len(ptrs):
+        raise _lib.JsdError("a state or adjoint buffer appears twice in the batch")
+
+
 class BatchPlan:
     def __init__(
         self,
@@ -36,6 +49,7 @@ class BatchPlan:
             raise _lib.JsdError("one state per plan")
         if adjoints is not None and len(adjoints) != len(plans):
             raise _lib.JsdError("one adjoint buffer per plan")
+        check_distinct(plans, states, adjoints)
         for p, s in zip(plans, states):
             p._check_state(s)
         self.plans = list(plans)

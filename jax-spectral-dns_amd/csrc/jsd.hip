@@ -3230,6 +3230,18 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_STATE, "run %d: the adjoint is built for linearised perturbation plans", r);
     }
+    // every run writes its own plan's workspaces inside each stage launch:
+    // a repeated plan or buffer would race
+    for (int q = 0; q < r; ++q)
+      if (plans[q] == p || states[q] == states[r] || (b->vjp && adjoints[q] == adjoints[r]) ||
+          (b->vjp && (adjoints[q] == states[r] || states[q] == adjoints[r]))) {
+        jsd_batch_destroy(b);
+        JSD_FAIL(JSD_ERR_ARG, "runs %d and %d share a plan or a buffer: each run needs its own", q, r);
+      }
+    if (b->vjp && adjoints[r] == states[r]) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_ARG, "run %d: state and adjoint share a buffer", r);
+    }
     b->plans.push_back(p);
     b->state.push_back((double2*)states[r]);
     b->adj.push_back(b->vjp ? (double2*)adjoints[r] : nullptr);

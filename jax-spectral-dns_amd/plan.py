@@ -170,6 +170,10 @@ class StepPlan:
 
     def nonlinear(self, vel_hat: torch.Tensor, fwd_hat: Optional[torch.Tensor] = None):
         self._check_state(vel_hat)
+        if fwd_hat is not None:
+            self._check_state(fwd_hat)
+        elif self.kind == _lib.KIND_DUAL and not self.linearise:
+            raise _lib.JsdError("a nonlinear dual plan needs the forward state")
         g = self.grid
         h_v = torch.empty((g.nxm, g.ny, g.nzm), dtype=torch.complex128, device=vel_hat.device)
         h_g = torch.empty_like(h_v)
@@ -186,9 +190,21 @@ class StepPlan:
         return out.value
 
     def ydiff(self, f: torch.Tensor, order: int = 1) -> torch.Tensor:
-        ncomp = f.shape[0] if f.dim() == 4 else 1
+        """y-derivative of one field (Nx_modes, Ny, Nz_modes) or of a stack
+        (k, Nx_modes, Ny, Nz_modes)."""
+        g = self.grid
+        one = (g.nxm, g.ny, g.nzm)
+        if f.dim() == 3 and tuple(f.shape) == one:
+            ncomp = 1
+        elif f.dim() == 4 and tuple(f.shape[1:]) == one and f.shape[0] >= 1:
+            ncomp = int(f.shape[0])
+        else:
+            raise _lib.JsdError("ydiff input shape %s: expected %s or (k,) + %s" % (tuple(f.shape), one, one))
+        if order not in (1, 2):
+            raise _lib.JsdError("ydiff order must be 1 or 2")
+        f = f.contiguous()
         out = torch.empty_like(f)
-        _lib.check(_lib.load().jsd_ydiff(self._h, _dev(f.contiguous()), _dev(out), ncomp, order, _stream()))
+        _lib.check(_lib.load().jsd_ydiff(self._h, _dev(f), _dev(out), ncomp, order, _stream()))
         return out
 
 

@@ -133,3 +133,25 @@ def test_batch_rejects_unsupported_plans():
     nonlin = StepPlan(g, 2000.0, 0.01, KIND_PERTURBATION, linearise=False)
     with pytest.raises(_lib.JsdError):
         BatchPlan([nonlin], [u])
+
+
+def test_library_rejects_repeated_plans_in_a_batch():
+    """jsd_batch_create itself (bypassing the Python check) rejects a plan or
+    a state buffer that appears twice: the runs would race on one plan's
+    workspaces."""
+    import ctypes
+
+    g = DM.make_grid((16, 33, 16), 2 * np.pi, 2 * np.pi)
+    p = StepPlan(g, 2000.0, 0.01, KIND_PERTURBATION, linearise=True)
+    q = StepPlan(g, 2000.0, 0.01, KIND_PERTURBATION, linearise=True)
+    u = torch.zeros(p.shape, dtype=torch.complex128, device=DEV)
+    w = torch.zeros_like(u)
+    lib = _lib.load()
+    vp = ctypes.c_void_p
+    st = vp(torch.cuda.current_stream().cuda_stream)
+    for plans, states in (([p, p], [u, w]), ([p, q], [u, u])):
+        hp = (vp * 2)(*[x._h for x in plans])
+        sp = (vp * 2)(*[vp(s.data_ptr()) for s in states])
+        h = vp()
+        rc = lib.jsd_batch_create(hp, sp, None, 2, st, ctypes.byref(h))
+        assert rc == -1 and not h.value

@@ -138,6 +138,40 @@ def test_nonlinear_perturbation(shape, lx, lz, linearise):
     assert rel(conv, rc) < 1e-12
 
 
+@pytest.mark.parametrize("coupling_term", [True, False])
+@pytest.mark.parametrize("shape", [(16, 33, 96), (16, 33, 160), (16, 32, 96)])
+def test_nonlinear_perturbation_z256(shape, coupling_term):
+    """The perturbation product on the z transform of length 256 (two y lines
+    per workgroup, base flow looked up per line, the second line masked when
+    Ny is odd); 161 stored kz take the unpruned first stage."""
+    g, d = pair(shape)
+    assert g.ppz == 256
+    plan = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=False, coupling_term=coupling_term)
+    u = perturbation_field(g, seed=14, energy=1e-2)
+    hv, hg, conv = plan.nonlinear(u.to(DEV))
+    base = O.base_flow_hat(d)
+    rv, rg, _, rc = O.nl_perturbation_skew(d, u, base, False, coupling_term)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(16, 33, 96), (16, 33, 160)])
+def test_nonlinear_dual_z256(shape):
+    """The dual product (12 packed transforms) on the z transform of length 256."""
+    g, d = pair(shape)
+    assert g.ppz == 256
+    fwd = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION)
+    plan = StepPlan(g, 3000.0, 0.01, KIND_DUAL, ops=fwd.ops, explicit_dt=-0.01, plan_re=-3000.0)
+    v = perturbation_field(g, seed=15, energy=1e-2)
+    uf = perturbation_field(g, seed=16, energy=1e-2)
+    hv, hg, conv = plan.nonlinear(v.to(DEV), uf.to(DEV))
+    rv, rg, _, rc = O.nl_dual_skew(d, v, O.base_flow_hat(d), uf, False)
+    assert rel(hv, rv) < 1e-12
+    assert rel(hg, rg) < 1e-12
+    assert rel(conv, rc) < 1e-12
+
+
 @pytest.mark.parametrize("linearise", [True, False])
 @pytest.mark.parametrize("shape,lx,lz", SHAPES)
 def test_nonlinear_dual(shape, lx, lz, linearise):
