@@ -141,6 +141,49 @@ int jsd_step_vjp_state(jsd_plan* plan, const void* state, void* grad, double dPd
  * Not for plans in a batch while it steps. */
 int jsd_set_base_flow(jsd_plan* plan, const double* base_u, const double* base_du, void* stream);
 
+/* One RK3/CN step like jsd_step (forward plans, fwd_hat NULL) or
+ * jsd_dual_step (dual plans) with the optional source term of
+ * get_source_term and the enforce_base switch.  source_hat (device,
+ * complex128, 3 components, NULL for none) is the step's source, used the
+ * way perform_runge_kutta_step uses it (navier_stokes.py:1697-1704, and the
+ * mean-flow block source_x_00 / source_z_00, navier_stokes.py:1461-1490):
+ * with Q = curl(source)_y, every kx row of h_v gains
+ * (i kz) Q[kx row 0] - (i kx_1) Q[kx row 2] and every kx row of h_g gains
+ * Q[kx row 1], as the reference's indexing of curl_source does; needs
+ * nxm >= 3.  enforce_base != 0 drops the (0,0) convective terms
+ * (navier_stokes.py:1581-1590; NavierStokesVelVortPerturbation enforce_base,
+ * navier_stokes_perturbation.py:418).  The first call allocates 1 spectral
+ * field plus 2 y-kz planes of workspace, kept by the plan. */
+int jsd_step_source(jsd_plan* plan, void* vel_hat, const void* fwd_hat, double dPdx, const void* source_hat,
+                    int enforce_base, void* stream);
+
+/* One RK substep s (0, 1, 2) with the nonlinear terms supplied by the
+ * caller: the nonlinear_update_fn plug-in (navier_stokes.py:324-332,
+ * navier_stokes.py:1688-1696) returns h_v, h_g (1 component each) and
+ * conv_ns (3 components) for vel_hat; the solve takes omega_y from vel_hat
+ * itself (the reference's terms all return vort = curl(vel_hat)).  The
+ * old terms are those of the previous substep call on this plan (s = 0
+ * reads none).  source_hat and enforce_base as in jsd_step_source. */
+int jsd_substep_terms(jsd_plan* plan, void* vel_hat, int s, double dPdx, const void* h_v, const void* h_g,
+                      const void* conv_ns, const void* source_hat, int enforce_base, void* stream);
+
+/* out = scale * laplacian(in) of ncomp spectral components
+ * (Domain.laplacian of a Fourier field, domain.py:671-673; the dual's
+ * dissipation source term, navier_stokes_perturbation_dual.py:962-988).
+ * in and out must not overlap. */
+int jsd_laplacian(jsd_plan* plan, const void* in, void* out, int ncomp, double scale, void* stream);
+
+/* acc (1 spectral component) += scale * sum_ij (d_j u_i)^2 with the complex
+ * squares of the spectral derivatives of vel_hat, the summand of the
+ * dissipation objective (get_dissipation_average,
+ * navier_stokes_perturbation_dual.py:1087-1104).  Uses the plan's
+ * y-derivative workspace: not between the substeps of a step. */
+int jsd_dissipation_accumulate(jsd_plan* plan, const void* vel_hat, void* acc, double scale, void* stream);
+
+/* *out (host) = volume integral of no_hat(f_hat) for one spectral
+ * component (PhysicalField.volume_integral, field.py:1456-1466). */
+int jsd_volume_integral(jsd_plan* plan, const void* f_hat, double* out, void* stream);
+
 /* out = scale * dE/d(vel_hat) on the real-pair view, E the energy of
  * jsd_energy (field.py:607-611). */
 int jsd_energy_grad(jsd_plan* plan, const void* vel_hat, void* out, double scale, void* stream);

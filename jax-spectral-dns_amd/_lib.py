@@ -73,6 +73,11 @@ EXPORTS = (
     "jsd_energy_grad",
     "jsd_profile",
     "jsd_set_base_flow",
+    "jsd_step_source",
+    "jsd_substep_terms",
+    "jsd_laplacian",
+    "jsd_dissipation_accumulate",
+    "jsd_volume_integral",
     "jsd_batch_create",
     "jsd_batch_destroy",
     "jsd_batch_step",
@@ -103,6 +108,11 @@ def load() -> ctypes.CDLL:
     lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
     lib.jsd_set_base_flow.argtypes = [vp, _dp, _dp, vp]
+    lib.jsd_step_source.argtypes = [vp, vp, vp, ctypes.c_double, vp, ctypes.c_int, vp]
+    lib.jsd_substep_terms.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, ctypes.c_int, vp]
+    lib.jsd_laplacian.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, vp]
+    lib.jsd_dissipation_accumulate.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    lib.jsd_volume_integral.argtypes = [vp, vp, _dp, vp]
     lib.jsd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, vp,
                                      ctypes.POINTER(vp)]
     lib.jsd_batch_destroy.argtypes = [vp]

@@ -1333,6 +1333,166 @@ __global__ __launch_bounds__(256) void k_add(const double2* __restrict__ A, doub
   if (e < n) B[e] = cadd(B[e], A[e]);
 }
 
+// ---------------------------------------------------------------------------
+// source term, enforce_base and externally supplied nonlinear terms
+// ---------------------------------------------------------------------------
+// The source term of one step (get_source_term, navier_stokes.py:1697-1704):
+// Q = curl(source)[1] = (i kz) s_x - (i kx) s_z; the reference then takes the
+// kx rows 0 and 2 of Q (curl_source[0], curl_source[2]), differentiates them
+// as (Ny, Nz) arrays that broadcast over kx, and keeps kx row 1 of the
+// difference, so every kx row of h_v gains
+//   hvs[l][j] = (i kz_j) Q[0][l][j] - (i kx_1) Q[2][l][j]
+// and every kx row of h_g gains hgs[l][j] = Q[1][l][j].  The (0,0) mean-flow
+// block gains source_x_00 = s_x[0][l][0] and source_z_00 = s_z[0][l][0]
+// (navier_stokes.py:1461-1490).  One thread per (l, j).
+__global__ __launch_bounds__(256) void k_source_prep(const double2* __restrict__ S, long cs,
+                                                     const double* __restrict__ kx, const double* __restrict__ kz,
+                                                     int N, int Z, double2* __restrict__ SRC,
+                                                     double2* __restrict__ S00) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= N * Z) return;
+  const int l = t / Z, j = t - (t / Z) * Z;
+  double2 q[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const size_t e = ((size_t)m * N + l) * Z + j;
+    q[m] = csub(cik(kz[j], S[e]), cik(kx[m], S[2 * cs + e]));
+  }
+  SRC[t] = csub(cik(kz[j], q[0]), cik(kx[1], q[2]));
+  SRC[(size_t)N * Z + t] = q[1];
+  if (j == 0) {
+    S00[l] = S[(size_t)l * Z];
+    S00[N + l] = S[2 * cs + (size_t)l * Z];
+  }
+}
+
+struct TermArgs {
+  double2* H;            // nonlinear term H (3 components), rewritten in place
+  long cs;
+  double2* HGS;          // out: the h_g addend the solve reads
+  const double2* HGC;    // coupling_term = False addend of this substep (or NULL)
+  const double2* EHV;    // external h_v, h_g, conv_ns (nonlinear_update_fn), or NULL
+  const double2* EHG;
+  const double2* ECV;
+  const double2* SRC;    // source addends hvs, hgs (N x Z each), or NULL
+  const double2* S00;    // source_x_00, source_z_00, or NULL
+  const double* kx;
+  const double* kz;
+  int M, N, Z, enforce_base;
+};
+
+// Rewrites H so that the solve, which forms h_v = -d_y(i kx H0 + i kz H2) -
+// k^2 H1, h_g = (i kz) H0 - (i kx) H2 + HGS and conv_ns(0,0) = -H(0,0), sees
+// the requested terms:
+//  * external terms: H0 = H2 = 0 off (0,0), H1 = -h_v / k^2, HGS = h_g,
+//    H(0,0) = -conv_ns(0,0);
+//  * a source: H1 -= hvs / k^2, HGS += hgs, H(0,0) += (source_x_00, source_z_00);
+//  * enforce_base: the (0,0) convective terms are dropped (navier_stokes.py:
+//    1581-1590) before the source is added.
+// The (0,0) pencil's h_v, h_g are unused (the mean-flow block replaces it).
+__global__ __launch_bounds__(256) void k_step_terms(TermArgs A) {
+  const size_t tot = (size_t)A.M * A.N * A.Z;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= tot) return;
+  const int Z = A.Z, N = A.N;
+  const int j = e % Z;
+  const int l = (e / Z) % N;
+  const int i = e / ((size_t)Z * N);
+  const double k2 = A.kx[i] * A.kx[i] + A.kz[j] * A.kz[j];
+  const bool z00 = (i == 0 && j == 0);
+  double2 h0, h1, h2, g;
+  const double2 zero = make_double2(0.0, 0.0);
+  if (A.EHV) {
+    const double2 hv = A.EHV[e];
+    h1 = k2 != 0.0 ? make_double2(-hv.x / k2, -hv.y / k2) : zero;
+    h0 = h2 = zero;
+    if (z00) {
+      const double2 c0 = A.ECV[e], c2 = A.ECV[2 * A.cs + e];
+      h0 = make_double2(-c0.x, -c0.y);
+      h2 = make_double2(-c2.x, -c2.y);
+    }
+    g = A.EHG[e];
+  } else {
+    h0 = A.H[e];
+    h1 = A.H[A.cs + e];
+    h2 = A.H[2 * A.cs + e];
+    g = A.HGC ? A.HGC[e] : zero;
+  }
+  if (A.SRC) {
+    const size_t t = (size_t)l * Z + j;
+    const double2 hvs = A.SRC[t];
+    if (k2 != 0.0) h1 = csub(h1, make_double2(hvs.x / k2, hvs.y / k2));
+    g = cadd(g, A.SRC[(size_t)N * Z + t]);
+  }
+  if (z00) {
+    if (A.enforce_base) h0 = h2 = zero;
+    if (A.S00) {
+      h0 = cadd(h0, A.S00[l]);
+      h2 = cadd(h2, A.S00[N + l]);
+    }
+  }
+  A.H[e] = h0;
+  A.H[A.cs + e] = h1;
+  A.H[2 * A.cs + e] = h2;
+  A.HGS[e] = g;
+}
+
+// out = scale ((i kx)^2 f + D^2 f + (i kz)^2 f) for ncomp components, with
+// out holding D^2 f on entry (Domain.laplacian, domain.py:671-673).
+__global__ __launch_bounds__(256) void k_laplacian_finish(const double2* __restrict__ F, double2* __restrict__ O,
+                                                          const double* __restrict__ kx,
+                                                          const double* __restrict__ kz, int M, int N, int Z,
+                                                          size_t tot, double scale) {
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= tot) return;
+  const size_t r = e % ((size_t)M * N * Z);
+  const int j = r % Z;
+  const int i = r / ((size_t)Z * N);
+  const double2 f = F[e], d2 = O[e];
+  const double ax = -(kx[i] * kx[i]), az = -(kz[j] * kz[j]);
+  const double2 s = cadd(cadd(cscale(f, ax), d2), cscale(f, az));
+  O[e] = cscale(s, scale);
+}
+
+// acc += scale * sum_i sum_j (d_j u_i)^2 with the complex squares of the
+// spectral derivatives, as the reference's dissipation sum does
+// (navier_stokes_perturbation_dual.py:1087-1104); DU holds d_y u_i.
+__global__ __launch_bounds__(256) void k_dissipation_acc(const double2* __restrict__ U, const double2* __restrict__ DU,
+                                                         long cs, const double* __restrict__ kx,
+                                                         const double* __restrict__ kz, int M, int N, int Z,
+                                                         double2* __restrict__ acc, double scale) {
+  const size_t tot = (size_t)M * N * Z;
+  const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= tot) return;
+  const int j = e % Z;
+  const int i = e / ((size_t)Z * N);
+  double2 s = acc[e];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 u = U[c * cs + e];
+    const double2 d[3] = {cik(kx[i], u), DU[c * cs + e], cik(kz[j], u)};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s = cadd(s, cscale(cmul(d[q], d[q]), scale));
+  }
+  acc[e] = s;
+}
+
+// Volume integral of no_hat(f) (field.py:1456-1466): the x-z sums of the
+// physical field are sqrt(Px Pz) s Re f(0, l, 0), then the y weights.
+__global__ void k_mean_integral(const double2* __restrict__ F, const double* __restrict__ wy, int N, int Z,
+                                double scale, double* __restrict__ out) {
+  __shared__ double red[256];
+  double a = 0.0;
+  for (int l = threadIdx.x; l < N; l += blockDim.x) a += wy[l] * F[(size_t)l * Z].x;
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
 // ===========================================================================
 // per-wavenumber implicit solve (navier_stokes.py:1248-1546)
 // ===========================================================================
@@ -1497,6 +1657,7 @@ struct jsd_plan {
   double *FF, *GF, *CWE, *RWE;  // E^-1 D and D E fragments, wall columns / rows in eigen coordinates
   double *VD2, *Dm;             // D^2 v_a parity tables with wall parts; D row-major (mean-flow derivatives)
   double2* HGC;                 // coupling_term = False: the h_g addend of the current substep
+  double2 *HGS, *SRC, *S00;     // source term / enforce_base / external terms (allocated on first use)
   int dual_cc;                  // dual plan, coupling_term = False: H_1 += hat(U' v_x) before the solve
   double c_impl[3];         // beta_s dt / Re of the Helmholtz operators
   double *EH_R, *EG_R, *LAMR;  // adjoint eigen form: full-row E and E^-1 (row-major), eigenvalue per row
@@ -2606,14 +2767,48 @@ static int dual_coupling_corr(jsd_plan* p, const double2* U, hipStream_t st) {
   return JSD_OK;
 }
 
+// What a substep takes beyond the library's own nonlinear term: externally
+// supplied h_v, h_g, conv_ns (the nonlinear_update_fn plug-in), a source term
+// (prepared by k_source_prep into p->SRC / p->S00) and enforce_base.
+struct StepExtras {
+  const double2 *hv, *hg, *conv;  // all set or all NULL
+  bool src;
+  int enforce_base;
+};
+
 static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPdx, hipStream_t st,
-                   bool have_du = false, bool write_du = false) {
-  int rc = nonlinear_H(p, U, UF, st, have_du);
-  if (rc) return rc;
-  rc = dual_coupling_corr(p, U, st);
-  if (rc) return rc;
+                   bool have_du = false, bool write_du = false, const StepExtras* x = nullptr) {
+  const bool ext = x && x->hv;
+  int rc;
+  if (!ext) {
+    rc = nonlinear_H(p, U, UF, st, have_du);
+    if (rc) return rc;
+    rc = dual_coupling_corr(p, U, st);
+    if (rc) return rc;
+  }
+  if (x) {
+    TermArgs T;
+    T.H = p->H;
+    T.cs = (long)p->fs;
+    T.HGS = p->HGS;
+    T.HGC = ext ? nullptr : p->HGC;
+    T.EHV = ext ? x->hv : nullptr;
+    T.EHG = ext ? x->hg : nullptr;
+    T.ECV = ext ? x->conv : nullptr;
+    T.SRC = x->src ? p->SRC : nullptr;
+    T.S00 = x->src ? p->S00 : nullptr;
+    T.kx = p->kx;
+    T.kz = p->kz;
+    T.M = p->M;
+    T.N = p->N;
+    T.Z = p->Z;
+    T.enforce_base = x->enforce_base;
+    hipLaunchKernelGGL(k_step_terms, dim3((unsigned)((p->fs + 255) / 256)), dim3(256), 0, st, T);
+    HIPCHK(hipGetLastError());
+  }
   const int nblk = ((solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
   SolveArgs S = solve_args(p, U, s, dPdx, roles_of(p));
+  if (x) S.HGC = p->HGS;
   if (write_du) {
     S.DUo = p->DU;
     S.du_full = (p->d.kind == JSD_KIND_NS) || !p->d.linearise;  // linearised steps need D v only
@@ -2662,6 +2857,102 @@ extern "C" int jsd_dual_step(jsd_plan* p, void* adj_hat, const void* fwd_hat, vo
     int rc = substep(p, (double2*)adj_hat, (const double2*)fwd_hat, s, 0.0, st);
     if (rc) return rc;
   }
+  return JSD_OK;
+}
+
+// Workspaces of the source term, enforce_base and external terms, allocated
+// on the first call that needs them and kept by the plan.
+static int extras_alloc(jsd_plan* p) {
+  if (p->HGS) return JSD_OK;
+  int rc = dalloc(p, (void**)&p->HGS, p->fs * sizeof(double2));
+  if (rc) return rc;
+  rc = dalloc(p, (void**)&p->SRC, 2 * (size_t)p->N * p->Z * sizeof(double2));
+  if (rc) return rc;
+  return dalloc(p, (void**)&p->S00, 2 * (size_t)p->N * sizeof(double2));
+}
+
+static int source_prep(jsd_plan* p, const double2* src, hipStream_t st) {
+  if (p->M < 3) JSD_FAIL(JSD_ERR_ARG, "a source term needs at least 3 stored kx (the reference reads kx rows 0..2)");
+  const int n = p->N * p->Z;
+  hipLaunchKernelGGL(k_source_prep, dim3((n + 255) / 256), dim3(256), 0, st, src, (long)p->fs, p->kx, p->kz, p->N,
+                     p->Z, p->SRC, p->S00);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+extern "C" int jsd_step_source(jsd_plan* p, void* vel_hat, const void* fwd_hat, double dPdx, const void* source_hat,
+                               int enforce_base, void* stream) {
+  if (!p || !vel_hat) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  const bool dual = p->d.kind == JSD_KIND_DUAL;
+  if (!dual && fwd_hat) JSD_FAIL(JSD_ERR_ARG, "a forward state is taken by dual plans only");
+  if (dual && !p->d.linearise && !fwd_hat) JSD_FAIL(JSD_ERR_ARG, "a nonlinear dual step needs the forward state");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = extras_alloc(p);
+  if (rc) return rc;
+  if (source_hat) {
+    rc = source_prep(p, (const double2*)source_hat, st);
+    if (rc) return rc;
+  }
+  StepExtras x{nullptr, nullptr, nullptr, source_hat != nullptr, enforce_base ? 1 : 0};
+  // the same derivative chain as jsd_step / jsd_dual_step
+  const bool chain = !dual && p->EVF != nullptr;
+  for (int s = 0; s < 3; ++s) {
+    rc = substep(p, (double2*)vel_hat, (const double2*)fwd_hat, s, dPdx, st, chain && s > 0, chain && s < 2, &x);
+    if (rc) return rc;
+  }
+  return JSD_OK;
+}
+
+extern "C" int jsd_substep_terms(jsd_plan* p, void* vel_hat, int s, double dPdx, const void* h_v, const void* h_g,
+                                 const void* conv_ns, const void* source_hat, int enforce_base, void* stream) {
+  if (!p || !vel_hat || !h_v || !h_g || !conv_ns || s < 0 || s > 2) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = extras_alloc(p);
+  if (rc) return rc;
+  if (source_hat) {
+    rc = source_prep(p, (const double2*)source_hat, st);
+    if (rc) return rc;
+  }
+  StepExtras x{(const double2*)h_v, (const double2*)h_g, (const double2*)conv_ns, source_hat != nullptr,
+               enforce_base ? 1 : 0};
+  return substep(p, (double2*)vel_hat, nullptr, s, dPdx, st, false, false, &x);
+}
+
+extern "C" int jsd_laplacian(jsd_plan* p, const void* in, void* out, int ncomp, double scale, void* stream) {
+  if (!p || !in || !out || ncomp < 1 || in == out) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_ygemm_plain(p, (const double2*)in, (double2*)out, ncomp, 2, st);
+  if (rc) return rc;
+  const size_t tot = (size_t)ncomp * p->fs;
+  hipLaunchKernelGGL(k_laplacian_finish, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, (const double2*)in,
+                     (double2*)out, p->kx, p->kz, p->M, p->N, p->Z, tot, scale);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+extern "C" int jsd_dissipation_accumulate(jsd_plan* p, const void* vel_hat, void* acc, double scale, void* stream) {
+  if (!p || !vel_hat || !acc) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_ygemm_plain(p, (const double2*)vel_hat, p->DU, 3, 1, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dissipation_acc, dim3((unsigned)((p->fs + 255) / 256)), dim3(256), 0, st,
+                     (const double2*)vel_hat, p->DU, (long)p->fs, p->kx, p->kz, p->M, p->N, p->Z, (double2*)acc,
+                     scale);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
+extern "C" int jsd_volume_integral(jsd_plan* p, const void* f_hat, double* out, void* stream) {
+  if (!p || !f_hat || !out) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  hipStream_t st = (hipStream_t)stream;
+  const double sl = (p->d.lx / (2 * M_PI)) * (p->d.lz / (2 * M_PI));
+  const double px = p->d.px, pz = p->d.pz;
+  const double scale = (p->d.lx / px) * (p->d.lz / pz) * sqrt(px * pz) * sl;
+  hipLaunchKernelGGL(k_mean_integral, dim3(1), dim3(256), 0, st, (const double2*)f_hat, p->wy, p->N, p->Z, scale,
+                     p->eout);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, p->eout, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return JSD_OK;
 }
 

@@ -41,6 +41,40 @@ class Equation:
         for f in fields:
             self.fields[f.get_name()] = [f]
 
+    @classmethod
+    def find_suitable_dt(
+        cls,
+        domain: Any,
+        max_cfl: float = 0.7,
+        U_max: tuple = (1.0, 1e-10, 1e-10),
+        end_time: Optional[float] = None,
+        safety_factor: float = 0.9,
+    ) -> float:
+        """CFL-limited time step (equation.py:114-162): the smallest
+        max_cfl * (extent / cells) / U_max over the directions times the
+        safety factor; with end_time, the number of steps is raised until
+        its median-factor split into outer x inner steps is balanced
+        (sqrt(n) - outer <= outer) and dt = end_time / n."""
+        from .navier_stokes import median_factor
+
+        dT = [
+            abs(max_cfl * (domain.get_extent(i) / domain.number_of_cells(i)) / U_max[i])
+            for i in domain.all_dimensions()
+        ]
+        dt = safety_factor * min(dT)
+        if end_time is None:
+            return dt
+        n = max(0, int(end_time / dt))
+
+        def bad(n: int) -> bool:
+            inner = median_factor(n)
+            outer = n // inner
+            return abs(np.sqrt(n)) - outer > outer
+
+        while bad(n):
+            n += 1
+        return end_time / n
+
     def get_dt(self) -> float:
         return self.dt
 

@@ -184,6 +184,18 @@ class FourierField(Field):
     def energy(self) -> float:
         return self.no_hat().energy()
 
+    def field_2d(self, direction: int, wavenumber: int = 0) -> "FourierField":
+        """Keep the modes of index wavenumber (and N - wavenumber) along
+        direction, zero the rest (field.py:2912-2935)."""
+        n = self.data.shape[direction]
+        keep = [wavenumber] if wavenumber == 0 else [wavenumber, n - wavenumber]
+        out = torch.zeros_like(self.data)
+        idx = [slice(None)] * self.data.dim()
+        for k in keep:
+            idx[direction] = slice(k, k + 1)
+            out[tuple(idx)] = out[tuple(idx)] + self.data[tuple(idx)]
+        return FourierField(self.physical_domain, out, self.name)
+
     @classmethod
     def FromFile(
         cls, domain: PhysicalDomain, filename: str, name: str = "field_hat", time_step: int = -1, device=None
@@ -238,6 +250,10 @@ class VectorField(Generic[T]):
 
     def energy(self) -> float:
         return float(sum(f.energy() for f in self.elements))
+
+    def field_2d(self, direction: int, wavenumber: int = 0) -> "VectorField":
+        """field.py:636-639."""
+        return VectorField([f.field_2d(direction, wavenumber) for f in self.elements], self.name)  # type: ignore[union-attr]
 
     def energy_with_other(self, other: "VectorField") -> float:
         return float(sum(a.energy_with_other(b) for a, b in zip(self.elements, other.elements)))  # type: ignore[union-attr]

@@ -20,6 +20,7 @@ from . import _lib
 from .domain import PhysicalDomain
 from .equation import Equation
 from .field import FourierField, PhysicalField, VectorField
+from .fixed_parameters import NavierStokesVelVortFixedParameters
 from .plan import StepPlan
 
 
@@ -48,6 +49,27 @@ class NavierStokesVelVort(Equation):
         self.linearise_flag = False
         self._plan: Optional[StepPlan] = None
         self._plan_params = params
+        # navier_stokes.py:335: only the perturbation equation may set it
+        self.enforce_base = False
+        # the nonlinear_update_fn plug-in (navier_stokes.py:324-332): by default the
+        # library's fused term; a callable (vel_hat, time_step) -> (h_v, h_g, vort,
+        # conv_ns) assigned here replaces it in every substep
+        self._library_nonlinear_update_fn = self._library_nonlinear
+        self.nonlinear_update_fn: Callable = self._library_nonlinear_update_fn
+        self.nse_fixed_parameters = NavierStokesVelVortFixedParameters(
+            physical_domain=pd,
+            poisson_mat=None,
+            rk_mats_rhs=None,
+            rk_mats_lhs_inv=None,
+            rk_rhs_inhom=None,
+            rk_mats_lhs_inv_inhom=None,
+            rk_mats_rhs_ns=None,
+            rk_mats_lhs_inv_ns=None,
+            Re_tau=self.Re_tau,
+            max_cfl=self.max_cfl,
+            u_max_over_u_tau=self.u_max_over_u_tau,
+            number_of_rk_steps=3,
+        )
 
     @staticmethod
     def _get_Re_tau(**params: Any) -> float:
@@ -73,6 +95,25 @@ class NavierStokesVelVort(Equation):
     def get_Re_tau(self) -> float:
         return self.Re_tau
 
+    def get_fixed_params(self) -> NavierStokesVelVortFixedParameters:
+        return self.nse_fixed_parameters
+
+    def get_source_term(self, _time_step: int) -> Optional[torch.Tensor]:
+        """Source hook (navier_stokes.py:1216-1217): None, or a complex128
+        (3, Nx_modes, Ny, Nz_modes) tensor added to the step's right-hand
+        side the way perform_runge_kutta_step adds it."""
+        return None
+
+    def _library_nonlinear(self, vel_hat: torch.Tensor, _time_step: int):
+        """The library's fused term as a nonlinear_update_fn: (h_v, h_g,
+        vort, conv_ns) of vel_hat (navier_stokes.py:48-73, 236-261)."""
+        h_v, h_g, conv = self.plan().nonlinear(self._state(vel_hat))
+        vort = self.get_domain().curl(self._state(vel_hat))
+        return h_v, h_g, vort, conv
+
+    def _uses_library_nonlinear(self) -> bool:
+        return self.nonlinear_update_fn is self._library_nonlinear_update_fn
+
     def get_u_max_over_u_tau(self) -> float:
         return self.u_max_over_u_tau
 
@@ -90,6 +131,50 @@ class NavierStokesVelVort(Equation):
             self._plan = StepPlan(g, self.Re_tau, self.get_dt(), self.kind, self.linearise_flag, self.u_max_over_u_tau)
         return self._plan
 
+    def plan_for_step(self, _time_step: int) -> StepPlan:
+        """The plan that advances step time_step (the perturbation equation
+        switches plans with linearise_switch / coupling_term_switch)."""
+        return self.plan()
+
+    def _advance(self, state: torch.Tensor, dPdx: float, time_step: int, nsteps: int) -> None:
+        """nsteps steps from time_step on, in place: one library call for the
+        plain step; step by step when a source term, enforce_base, a
+        nonlinear_update_fn plug-in or a per-step plan switch is in use."""
+        plain = self._uses_library_nonlinear() and not self.enforce_base and not self._has_step_hooks()
+        if plain and self._plans_uniform(time_step, nsteps):
+            self.plan_for_step(time_step).step(state, dPdx, nsteps)
+            return
+        for t in range(time_step, time_step + nsteps):
+            self._one_step(state, dPdx, t)
+
+    def _has_step_hooks(self) -> bool:
+        """get_source_term overridden (in a subclass or on the instance)."""
+        return getattr(self.get_source_term, "__func__", None) is not NavierStokesVelVort.get_source_term
+
+    def _plans_uniform(self, time_step: int, nsteps: int) -> bool:
+        return True
+
+    def _one_step(self, state: torch.Tensor, dPdx: float, time_step: int, fwd_hat: Optional[torch.Tensor] = None):
+        plan = self.plan_for_step(time_step)
+        src = self.get_source_term(time_step)
+        if src is not None:
+            src = self._state(src)
+        if self._uses_library_nonlinear():
+            if src is None and not self.enforce_base:
+                if fwd_hat is None:
+                    plan.step(state, dPdx, 1)
+                else:
+                    plan.dual_step(state, fwd_hat)
+            else:
+                plan.step_source(state, dPdx, src, self.enforce_base, fwd_hat)
+            return
+        # the plug-in: terms from the user's function in every substep
+        for s in range(3):
+            h_v, h_g, _vort, conv = self.nonlinear_update_fn(state, time_step)
+            plan.substep_terms(
+                state, s, self._state(h_v), self._state(h_g), self._state(conv), dPdx, src, self.enforce_base
+            )
+
     def _state(self, vel_hat_data: Optional[torch.Tensor]) -> torch.Tensor:
         if vel_hat_data is None:
             vel_hat_data = self.get_latest_field("velocity_hat").get_data()
@@ -103,7 +188,8 @@ class NavierStokesVelVort(Equation):
         explicit = vel_hat_data is not None
         state = self._state(vel_hat_data).clone()
         dp = self.dPdx if dPdx is None else dPdx
-        self.plan().step(state, dp, 1)
+        t = self.time_step if time_step is None else int(time_step)
+        self._advance(state, dp, t, 1)
         if not explicit:
             vh = VectorField.FromData(FourierField, self.physical_domain, state, "velocity_hat")
             vh.set_time_step(self.get_latest_field("velocity_hat").get_time_step() + 1)
@@ -129,9 +215,8 @@ class NavierStokesVelVort(Equation):
             outer = n // inner if n > 0 else 0
         self.number_of_inner_steps, self.number_of_outer_steps, self.number_of_time_steps_ = inner, outer, n
         traj = [u0]
-        plan = self.plan()
-        for _ in range(outer):
-            plan.step(u, dp, inner)
+        for o in range(outer):
+            self._advance(u, dp, o * inner, inner)
             if self.write_intermediate_output or self.write_entire_output:
                 traj.append(u.clone())
         final = VectorField.FromData(FourierField, self.physical_domain, u.clone(), "velocity_hat")

@@ -52,5 +52,6 @@ class NavierStokesVelVortPerturbationInstationaryDual(NavierStokesVelVortPerturb
             fplan.step(u, 0.0, 1)
 
     def _dual_step(self, v: torch.Tensor, t: int, fwd_state: Optional[torch.Tensor]) -> None:
-        self._plan.set_base_flow(self.vel_base_profile(t))
-        self._plan.dual_step(v, fwd_state)
+        plan = self.plan_for_step(t)
+        plan.set_base_flow(self.vel_base_profile(t))
+        plan.dual_step(v, fwd_state)

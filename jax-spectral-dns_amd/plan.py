@@ -51,6 +51,7 @@ class StepPlan:
         adjoint: bool = False,
         eigen_solve: bool = True,
         coupling_term: bool = True,
+        base_profile: Optional[np.ndarray] = None,
     ):
         if not torch.cuda.is_available():
             raise _lib.JsdError("no GPU visible: the channel-flow step runs only through the HIP library")
@@ -69,8 +70,13 @@ class StepPlan:
             self._keep.append(a)
             return a
 
-        base_u = arr(u_max * (1 - g.y**2))
+        # base-flow profile U(y) of perturbation / dual plans (default: laminar
+        # u_max (1 - y^2)) and U'(y) = D U
+        base_u = arr(u_max * (1 - g.y**2) if base_profile is None else np.asarray(base_profile, dtype=np.float64))
+        if base_u.shape != (g.ny,):
+            raise ValueError("base profile needs %d values" % g.ny)
         base_du = arr(g.D @ base_u)
+        self.base_profile = base_u.copy()
         desc = _lib.PlanDesc()
         desc.nxm, desc.ny, desc.nzm = g.nxm, g.ny, g.nzm
         desc.px, desc.pz, desc.ppx, desc.ppz = g.px, g.pz, g.ppx, g.ppz
@@ -160,6 +166,92 @@ class StepPlan:
         self._check_state(vel_hat)
         _lib.check(_lib.load().jsd_step(self._h, _dev(vel_hat), float(dPdx), int(nsteps), _stream()))
         return vel_hat
+
+    def step_source(
+        self,
+        vel_hat: torch.Tensor,
+        dPdx: float = 0.0,
+        source_hat: Optional[torch.Tensor] = None,
+        enforce_base: bool = False,
+        fwd_hat: Optional[torch.Tensor] = None,
+    ) -> torch.Tensor:
+        """One step (a dual step for dual plans, with the forward state
+        fwd_hat) with the source term source_hat (3 components) and the
+        enforce_base switch; in place."""
+        self._check_state(vel_hat)
+        if source_hat is not None:
+            self._check_state(source_hat)
+        if fwd_hat is not None:
+            self._check_state(fwd_hat)
+        _lib.check(
+            _lib.load().jsd_step_source(
+                self._h, _dev(vel_hat), _dev(fwd_hat), float(dPdx), _dev(source_hat), int(bool(enforce_base)), _stream()
+            )
+        )
+        return vel_hat
+
+    def substep_terms(
+        self,
+        vel_hat: torch.Tensor,
+        s: int,
+        h_v: torch.Tensor,
+        h_g: torch.Tensor,
+        conv_ns: torch.Tensor,
+        dPdx: float = 0.0,
+        source_hat: Optional[torch.Tensor] = None,
+        enforce_base: bool = False,
+    ) -> torch.Tensor:
+        """RK substep s with externally supplied nonlinear terms (the
+        nonlinear_update_fn plug-in); in place."""
+        g = self.grid
+        self._check_state(vel_hat)
+        self._check_state(conv_ns)
+        for t in (h_v, h_g):
+            if tuple(t.shape) != (g.nxm, g.ny, g.nzm):
+                raise _lib.JsdError("h_v / h_g shape %s != %s" % (tuple(t.shape), (g.nxm, g.ny, g.nzm)))
+        if source_hat is not None:
+            self._check_state(source_hat)
+        _lib.check(
+            _lib.load().jsd_substep_terms(
+                self._h, _dev(vel_hat), int(s), float(dPdx), _dev(h_v), _dev(h_g), _dev(conv_ns), _dev(source_hat),
+                int(bool(enforce_base)), _stream(),
+            )
+        )
+        return vel_hat
+
+    def laplacian(self, f: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        """scale * laplacian of a spectral field or a stack of them."""
+        g = self.grid
+        one = (g.nxm, g.ny, g.nzm)
+        if tuple(f.shape) == one:
+            ncomp = 1
+        elif f.dim() == 4 and tuple(f.shape[1:]) == one:
+            ncomp = int(f.shape[0])
+        else:
+            raise _lib.JsdError("laplacian input shape %s" % (tuple(f.shape),))
+        f = f.contiguous()
+        out = torch.empty_like(f)
+        _lib.check(_lib.load().jsd_laplacian(self._h, _dev(f), _dev(out), ncomp, float(scale), _stream()))
+        return out
+
+    def dissipation_accumulate(self, vel_hat: torch.Tensor, acc: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+        """acc += scale * sum_ij (d_j u_i)^2 (complex squares of the spectral
+        derivatives); in place on acc."""
+        g = self.grid
+        self._check_state(vel_hat)
+        if tuple(acc.shape) != (g.nxm, g.ny, g.nzm):
+            raise _lib.JsdError("accumulator shape %s" % (tuple(acc.shape),))
+        _lib.check(_lib.load().jsd_dissipation_accumulate(self._h, _dev(vel_hat), _dev(acc), float(scale), _stream()))
+        return acc
+
+    def volume_integral(self, f_hat: torch.Tensor) -> float:
+        """Volume integral of no_hat(f_hat) for one spectral component."""
+        g = self.grid
+        if tuple(f_hat.shape) != (g.nxm, g.ny, g.nzm):
+            raise _lib.JsdError("field shape %s" % (tuple(f_hat.shape),))
+        out = ctypes.c_double(0.0)
+        _lib.check(_lib.load().jsd_volume_integral(self._h, _dev(f_hat.contiguous()), ctypes.byref(out), _stream()))
+        return out.value
 
     def dual_step(self, adj_hat: torch.Tensor, fwd_hat: Optional[torch.Tensor]) -> torch.Tensor:
         self._check_state(adj_hat)

@@ -587,6 +587,23 @@ class StepParams:
     dt: float  # the explicit-term time step (negative for the dual)
     dPdx: float = 0.0
     nonlinear: Optional[Callable[[torch.Tensor, int], Tuple[torch.Tensor, ...]]] = None
+    # get_source_term(time_step) -> None | complex (3, Nx, Ny, Nz) (navier_stokes.py:1216-1217)
+    source: Optional[Callable[[int], Optional[torch.Tensor]]] = None
+    # enforce_base: the (0,0) convective terms are dropped (navier_stokes.py:1581-1590)
+    enforce_base: bool = False
+
+
+def source_step_terms(dom: Dom, source_hat: torch.Tensor, h_v: torch.Tensor, h_g: torch.Tensor):
+    """The source term's addends to h_v, h_g as the reference writes them
+    (navier_stokes.py:1697-1704): curl_source = curl(source)[1] is the y
+    component (Nx, Ny, Nz); curl_source[0] and curl_source[2] index its kx
+    rows 0 and 2, whose (Ny, Nz) derivatives broadcast over kx, and the
+    trailing [1] keeps kx row 1 of their difference, which again broadcasts
+    over every kx row of h_v; h_g gains kx row 1 of curl_source."""
+    curl_source = curl(dom, source_hat)[1]
+    h_v = h_v + (diff(dom, curl_source[0], 2) - diff(dom, curl_source[2], 0))[1]
+    h_g = h_g + curl_source[1]
+    return h_v, h_g
 
 
 def _bc_pencil(x: torch.Tensor) -> torch.Tensor:
@@ -605,6 +622,7 @@ def solve_wavenumbers(
     conv: torch.Tensor,
     conv_old: torch.Tensor,
     vel_hat: torch.Tensor,
+    source_hat: Optional[torch.Tensor] = None,
 ) -> torch.Tensor:
     """All per-wavenumber solves of one RK substep, vectorised over (kx, kz).
     perform_single_rk_step_for_single_wavenumber, navier_stokes.py:1228-1548,
@@ -657,8 +675,20 @@ def solve_wavenumbers(
     dpdx = sp.dPdx * ((dom.px * (2 * np.pi / dom.lx) ** 2) * (dom.pz * (2 * np.pi / dom.lz) ** 2)) ** 0.5
     v_hat00 = torch.cat([vel_hat[0][0, :, 0], vel_hat[2][0, :, 0]])
     ones = torch.ones(n, dtype=C128)
-    n_new = torch.cat([-conv[0][0, :, 0], -conv[2][0, :, 0]]) + torch.cat([-dpdx * ones, 0 * ones])
-    n_old = torch.cat([-conv_old[0][0, :, 0], -conv_old[2][0, :, 0]]) + torch.cat([-dpdx * ones, 0 * ones])
+    # source_x_00, source_z_00 (navier_stokes.py:1461-1490): the same current
+    # source enters the new and the old term
+    if source_hat is None:
+        sx00 = torch.zeros(n, dtype=C128)
+        sz00 = torch.zeros(n, dtype=C128)
+    else:
+        sx00 = source_hat[0][0, :, 0]
+        sz00 = source_hat[2][0, :, 0]
+    c00, c20 = conv[0][0, :, 0], conv[2][0, :, 0]
+    c00o, c20o = conv_old[0][0, :, 0], conv_old[2][0, :, 0]
+    if sp.enforce_base:  # navier_stokes.py:1581-1590
+        c00, c20, c00o, c20o = (torch.zeros_like(c00) for _ in range(4))
+    n_new = torch.cat([-c00, -c20]) + torch.cat([-dpdx * ones + sx00, 0 * ones + sz00])
+    n_old = torch.cat([-c00o, -c20o]) + torch.cat([-dpdx * ones + sx00, 0 * ones + sz00])
     lhs00 = dom.t_("lhs_inv_ns" + s, ops.lhs_inv_ns[step])
     rhs00 = dom.t_("rhs_ns" + s, ops.rhs_ns[step])
     v00 = lhs00 @ (rhs00 @ v_hat00 + (dt * gamma) * n_new + (dt * xi) * n_old)
@@ -676,13 +706,18 @@ def rk_step(sp: StepParams, vel_hat: torch.Tensor, time_step: int) -> torch.Tens
     h_v_old = h_g_old = conv_old = None
     for step in range(3):
         h_v, h_g, vort, conv = sp.nonlinear(vel_hat, time_step)  # type: ignore[misc]
+        source_hat = sp.source(time_step) if sp.source is not None else None
+        if source_hat is not None:  # navier_stokes.py:1697-1704
+            h_v, h_g = source_step_terms(dom, source_hat, h_v, h_g)
         if step == 0:
             h_v_old = torch.zeros_like(h_v)
             h_g_old = torch.zeros_like(h_g)
             conv_old = torch.zeros_like(conv)
         v1 = vel_hat[1]
         v_1_lap = diff(dom, v1, 0, 2) + diff(dom, v1, 1, 2) + diff(dom, v1, 2, 2)
-        new = solve_wavenumbers(sp, step, v_1_lap, vort[1], h_v, h_g, h_v_old, h_g_old, conv, conv_old, vel_hat)
+        new = solve_wavenumbers(
+            sp, step, v_1_lap, vort[1], h_v, h_g, h_v_old, h_g_old, conv, conv_old, vel_hat, source_hat
+        )
         h_v_old, h_g_old, conv_old = h_v, h_g, conv
         vel_hat = torch.stack([update_bc(new[i]) for i in range(3)])
     return vel_hat
@@ -803,10 +838,13 @@ def make_perturbation(
     coupling_term: bool = True,
     u_max_over_u_tau: float = 1.0,
     ops: Optional[Operators] = None,
+    base_hat: Optional[torch.Tensor] = None,
 ) -> StepParams:
+    """velocity_base_hat defaults to the laminar profile
+    (navier_stokes_perturbation.py:351-380); any base_hat may be given."""
     re_tau = re / u_max_over_u_tau
     ops = ops if ops is not None else build_operators(dom, re_tau, dt)
-    base = base_flow_hat(dom, u_max_over_u_tau)
+    base = base_flow_hat(dom, u_max_over_u_tau) if base_hat is None else base_hat
     sp = StepParams(dom, ops, dt, 0.0)
     sp.nonlinear = lambda vh, t: nl_perturbation_skew(dom, vh, base, linearise, coupling_term)
     sp.base_hat = base  # type: ignore[attr-defined]
@@ -847,20 +885,91 @@ def run(sp: StepParams, vel_hat: torch.Tensor, n_steps: int, keep: bool = False,
     return (vel_hat, hist) if keep else vel_hat
 
 
-def dual_gradient(fwd: StepParams, u0_hat: torch.Tensor, n_steps: int, linearise: bool, checkpoint: Optional[int] = None):
-    """Adjoint gain gradient (u0 - v0)/E0 with v(T) = -u(T).
+def laplacian(dom: Dom, fh: torch.Tensor) -> torch.Tensor:
+    """Domain.laplacian (domain.py:671-673)."""
+    return diff(dom, fh, 0, 2) + diff(dom, fh, 1, 2) + diff(dom, fh, 2, 2)
+
+
+def field_2d(fh: torch.Tensor, axis: int) -> torch.Tensor:
+    """FourierField.field_2d(direction) with wavenumber 0 (field.py:2912-2935):
+    keep index 0 along ``axis`` (0: kx, 2: kz of a (Nx, Ny, Nz) field),
+    zero the rest."""
+    out = torch.zeros_like(fh)
+    idx = [slice(None)] * fh.dim()
+    idx[axis] = slice(0, 1)
+    out[tuple(idx)] = fh[tuple(idx)]
+    return out
+
+
+def dual_initial_condition(uT: torch.Tensor, mode: str) -> torch.Tensor:
+    """navier_stokes_perturbation_dual.py:903-935."""
+    if mode == "gain":
+        return -1 * uT
+    if mode == "gain_3d":
+        return -1 * (uT - torch.stack([field_2d(uT[i], 0) for i in range(3)]))
+    if mode == "gain_no_00":
+        return -1 * (uT - torch.stack([field_2d(field_2d(uT[i], 0), 2) for i in range(3)]))
+    if mode == "dissipation":
+        return 0 * uT
+    raise ValueError(mode)
+
+
+def objective(dom: Dom, mode: str, u0: torch.Tensor, uT: torch.Tensor, hist: Optional[List[torch.Tensor]], re_tau: float):
+    """get_objective_fun (navier_stokes_perturbation_dual.py:941-957, 1058-1104):
+    the gains E(u(T) - part)/E(u(0)); the dissipation average
+    volume_integral(no_hat(sum_states sum_ij diff(u_i, j)**2)) / |Re_tau n_states|
+    with the squares of the spectral coefficients, as there."""
+    if mode == "gain":
+        return energy(dom, uT) / energy(dom, u0)
+    if mode == "gain_3d":
+        return energy(dom, uT - torch.stack([field_2d(uT[i], 0) for i in range(3)])) / energy(dom, u0)
+    if mode == "gain_no_00":
+        return energy(dom, uT - torch.stack([field_2d(field_2d(uT[i], 0), 2) for i in range(3)])) / energy(dom, u0)
+    assert hist is not None
+    acc = torch.zeros_like(u0[0])
+    for u in hist:
+        for i in range(3):
+            for j in range(3):
+                acc = acc + diff(dom, u[i], j) ** 2
+    acc = acc / abs(re_tau * len(hist))
+    return volume_integral(dom, field_no_hat(dom, acc))
+
+
+def dual_gradient(
+    fwd: StepParams,
+    u0_hat: torch.Tensor,
+    n_steps: int,
+    linearise: bool,
+    checkpoint: Optional[int] = None,
+    mode: str = "gain",
+    re_tau: Optional[float] = None,
+):
+    """Adjoint gradient (u0 - v0)/E0 with the dual started from
+    dual_initial_condition(u(T), mode) (v(T) = -u(T) for the gain).
     run_forward_calculation / solve_scan / get_grad,
     navier_stokes_perturbation_dual.py:690-1120.  With ``checkpoint`` set,
     only every checkpoint-th forward state is stored and each chunk is
     recomputed before its adjoint sweep (navier_stokes_perturbation_dual.py:730-757,
-    :990-1035)."""
+    :990-1035).  The dissipation mode adds the source
+    -laplacian(u(t)) / |Re_dual T| at dual step t (get_source_term,
+    navier_stokes_perturbation_dual.py:962-988; Re_dual = -re_tau,
+    T = n_steps |dt|) and returns the dissipation average as its objective."""
     dom = fwd.dom
+    if mode == "dissipation" and checkpoint is not None:
+        raise ValueError("the oracle's dissipation mode keeps the whole history")
     if checkpoint is None:
         uT, hist = run(fwd, u0_hat, n_steps, keep=True)
         dual = make_dual(fwd, lambda t: hist[-1 - t], linearise)
-        v = -uT
+        if mode == "dissipation":
+            assert re_tau is not None
+            scale = -1 / abs(-re_tau * (-n_steps * abs(fwd.dt)))
+            dual.source = lambda t: scale * torch.stack([laplacian(dom, hist[-1 - t][i]) for i in range(3)])
+        v = dual_initial_condition(uT, mode)
         for t in range(n_steps):
             v = rk_step(dual, v, t)
+        e0 = energy(dom, u0_hat)
+        obj = objective(dom, mode, u0_hat, uT, hist, re_tau if re_tau is not None else 1.0)
+        return (u0_hat - v) / e0, obj, v
     else:
         inner = checkpoint
         outer = n_steps // inner
@@ -870,7 +979,7 @@ def dual_gradient(fwd: StepParams, u0_hat: torch.Tensor, n_steps: int, linearise
             u = run(fwd, u, inner)
             ckpt.append(u)
         uT = u
-        v = -uT
+        v = dual_initial_condition(uT, mode)
         cur: List[torch.Tensor] = []
         start = [0]
 
@@ -885,8 +994,7 @@ def dual_gradient(fwd: StepParams, u0_hat: torch.Tensor, n_steps: int, linearise
             for t in range(o * inner, (o + 1) * inner):
                 v = rk_step(dual, v, t)
     e0 = energy(dom, u0_hat)
-    gain = energy(dom, uT) / e0
-    return (u0_hat - v) / e0, gain, v
+    return (u0_hat - v) / e0, objective(dom, mode, u0_hat, uT, None, 1.0), v
 
 
 # ---------------------------------------------------------------------------

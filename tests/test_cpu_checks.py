@@ -57,3 +57,29 @@ def test_batch_rejects_repeated_plans_and_buffers():
         check_distinct([a, b], [s[0], s[0]])
     with pytest.raises(_lib.JsdError):
         check_distinct([a, b], s, [s[1], torch.zeros(4)])
+
+
+def test_find_suitable_dt():
+    """Equation.find_suitable_dt (equation.py:114-162): CFL bound over the
+    directions, then a step count whose median-factor split is balanced."""
+    import numpy as np
+
+    from jax_spectral_dns_amd.domain import PhysicalDomain
+    from jax_spectral_dns_amd.equation import Equation
+    from jax_spectral_dns_amd.navier_stokes import median_factor
+
+    pd = PhysicalDomain.create((24, 65, 4), (True, False, True), scale_factors=(2 * np.pi, 1.0, 2 * np.pi))
+    dt = Equation.find_suitable_dt(pd, max_cfl=0.7)
+    want = 0.9 * min(0.7 * (2 * np.pi / 38) / 1.0, 0.7 * (2.0 / 65) / 1e-10, 0.7 * (2 * np.pi / 8) / 1e-10)
+    assert dt == want
+    dt_T = Equation.find_suitable_dt(pd, max_cfl=0.7, end_time=15.0)
+    n = round(15.0 / dt_T)
+    assert abs(n * dt_T - 15.0) < 1e-12
+    assert n >= int(15.0 / dt)
+    inner = median_factor(n)
+    outer = n // inner
+    assert not (np.sqrt(n) - outer > outer)
+    # the first count that balances: every smaller count from int(T/dt) is unbalanced
+    for m in range(int(15.0 / dt), n):
+        i = median_factor(m)
+        assert np.sqrt(m) - m // i > m // i
