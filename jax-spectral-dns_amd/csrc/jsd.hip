@@ -1382,14 +1382,16 @@ struct TermArgs {
 };
 
 // Rewrites H so that the solve, which forms h_v = -d_y(i kx H0 + i kz H2) -
-// k^2 H1, h_g = (i kz) H0 - (i kx) H2 + HGS and conv_ns(0,0) = -H(0,0), sees
-// the requested terms:
+// k^2 H1 (for the (0,0) pencil: h_v = H1, the h1_00 mode), h_g = (i kz) H0 -
+// (i kx) H2 + HGS and conv_ns(0,0) = -H(0,0), sees the requested terms:
 //  * external terms: H0 = H2 = 0 off (0,0), H1 = -h_v / k^2, HGS = h_g,
-//    H(0,0) = -conv_ns(0,0);
-//  * a source: H1 -= hvs / k^2, HGS += hgs, H(0,0) += (source_x_00, source_z_00);
+//    H(0,0) = -conv_ns(0,0), H1(0,0) = h_v(0,0);
+//  * a source: H1 -= hvs / k^2 (H1 += hvs at (0,0), whose v the reference
+//    solves with the source's h_v too), HGS += hgs,
+//    H(0,0) += (source_x_00, source_z_00);
 //  * enforce_base: the (0,0) convective terms are dropped (navier_stokes.py:
 //    1581-1590) before the source is added.
-// The (0,0) pencil's h_v, h_g are unused (the mean-flow block replaces it).
+// The (0,0) pencil's h_g is unused (the mean-flow block replaces u, w there).
 __global__ __launch_bounds__(256) void k_step_terms(TermArgs A) {
   const size_t tot = (size_t)A.M * A.N * A.Z;
   const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -1404,7 +1406,8 @@ __global__ __launch_bounds__(256) void k_step_terms(TermArgs A) {
   const double2 zero = make_double2(0.0, 0.0);
   if (A.EHV) {
     const double2 hv = A.EHV[e];
-    h1 = k2 != 0.0 ? make_double2(-hv.x / k2, -hv.y / k2) : zero;
+    // the (0,0) pencil's H1 carries its h_v itself (the solve's h1_00 mode)
+    h1 = z00 ? hv : (k2 != 0.0 ? make_double2(-hv.x / k2, -hv.y / k2) : zero);
     h0 = h2 = zero;
     if (z00) {
       const double2 c0 = A.ECV[e], c2 = A.ECV[2 * A.cs + e];
@@ -1414,14 +1417,17 @@ __global__ __launch_bounds__(256) void k_step_terms(TermArgs A) {
     g = A.EHG[e];
   } else {
     h0 = A.H[e];
-    h1 = A.H[A.cs + e];
+    h1 = z00 ? zero : A.H[A.cs + e];  // h_v of the (0,0) pencil is zero without a source
     h2 = A.H[2 * A.cs + e];
     g = A.HGC ? A.HGC[e] : zero;
   }
   if (A.SRC) {
     const size_t t = (size_t)l * Z + j;
     const double2 hvs = A.SRC[t];
-    if (k2 != 0.0) h1 = csub(h1, make_double2(hvs.x / k2, hvs.y / k2));
+    if (z00)
+      h1 = cadd(h1, hvs);
+    else if (k2 != 0.0)
+      h1 = csub(h1, make_double2(hvs.x / k2, hvs.y / k2));
     g = cadd(g, A.SRC[(size_t)N * Z + t]);
   }
   if (z00) {
@@ -2752,6 +2758,7 @@ static SolveArgs solve_args(const jsd_plan* p, double2* U, int s, double dPdx, c
   S.diag_alpha = RK_ALP[s] * fabs(dt) / fabs(p->d.re_tau);
   S.dpdx = dPdx * p->dpdx_scale;
   S.use_old = s > 0;
+  S.h1_00 = 0;
   return S;
 }
 
@@ -2808,7 +2815,10 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   }
   const int nblk = ((solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
   SolveArgs S = solve_args(p, U, s, dPdx, roles_of(p));
-  if (x) S.HGC = p->HGS;
+  if (x) {
+    S.HGC = p->HGS;
+    S.h1_00 = 1;
+  }
   if (write_du) {
     S.DUo = p->DU;
     S.du_full = (p->d.kind == JSD_KIND_NS) || !p->d.linearise;  // linearised steps need D v only
@@ -2863,6 +2873,7 @@ extern "C" int jsd_dual_step(jsd_plan* p, void* adj_hat, const void* fwd_hat, vo
 // Workspaces of the source term, enforce_base and external terms, allocated
 // on the first call that needs them and kept by the plan.
 static int extras_alloc(jsd_plan* p) {
+  if (!p->EVF) JSD_FAIL(JSD_ERR_STATE, "source terms, enforce_base and external terms need the eigen-form solve");
   if (p->HGS) return JSD_OK;
   int rc = dalloc(p, (void**)&p->HGS, p->fs * sizeof(double2));
   if (rc) return rc;

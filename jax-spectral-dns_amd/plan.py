@@ -211,6 +211,9 @@ class StepPlan:
                 raise _lib.JsdError("h_v / h_g shape %s != %s" % (tuple(t.shape), (g.nxm, g.ny, g.nzm)))
         if source_hat is not None:
             self._check_state(source_hat)
+        h_v, h_g, conv_ns = h_v.contiguous(), h_g.contiguous(), conv_ns.contiguous()
+        if source_hat is not None:
+            source_hat = source_hat.contiguous()
         _lib.check(
             _lib.load().jsd_substep_terms(
                 self._h, _dev(vel_hat), int(s), float(dPdx), _dev(h_v), _dev(h_g), _dev(conv_ns), _dev(source_hat),
