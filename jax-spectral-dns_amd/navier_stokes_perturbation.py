@@ -74,18 +74,18 @@ class NavierStokesVelVortPerturbation(NavierStokesVelVort):
         if self.linearise_switch is not None:
             assert params.get("linearise") is None, "need to either pass linearise or linearise_switch."
             sw = float(self.linearise_switch)
-            self.linearise = lambda t: t < n * sw
+            self.linearise = lambda t=0: t < n * sw
         else:
             lin = bool(params.get("linearise", False) or False)
-            self.linearise = lambda _t: lin
+            self.linearise = lambda _t=0: lin
         if self.coupling_term_switch is not None:
             assert params.get("coupling_term") is None, "need to either pass coupling_term or coupling_term_switch."
             csw = float(self.coupling_term_switch)
-            self.coupling_term = lambda t: not (t < n * csw)
+            self.coupling_term = lambda t=0: not (t < n * csw)
         else:
             cpl = params.get("coupling_term", True)
             cpl = True if cpl is None else bool(cpl)
-            self.coupling_term = lambda _t: cpl
+            self.coupling_term = lambda _t=0: cpl
         self.linearise_flag = bool(self.linearise(0))
         self.coupling_term_flag = bool(self.coupling_term(0))
         self.enforce_base = bool(params.get("enforce_base", False))

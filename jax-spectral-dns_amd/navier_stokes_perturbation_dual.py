@@ -50,18 +50,18 @@ class NavierStokesVelVortPerturbationDual:
         n_sw = len(np.arange(0, abs(self.end_time), abs(self.dt))) if self.end_time else 0
         if getattr(fwd, "linearise_switch", None) is not None:
             sw = float(fwd.linearise_switch)
-            self.linearise = lambda t: t + 1 >= n_sw * (1.0 - sw)
+            self.linearise = lambda t=0: t + 1 >= n_sw * (1.0 - sw)
         else:
             lin = bool(fwd.linearise(0))
-            self.linearise = lambda _t: lin
+            self.linearise = lambda _t=0: lin
         if getattr(fwd, "coupling_term_switch", None) is not None:
             csw = float(fwd.coupling_term_switch)
-            self.coupling_term = lambda t: not (t + 1 >= n_sw * csw)
+            self.coupling_term = lambda t=0: not (t + 1 >= n_sw * csw)
         else:
             # coupling_term = False: h_v gains (d_x^2 + d_z^2) hat(U' v_x)
             # (navier_stokes_perturbation_dual.py:269-278)
             cpl = bool(params.get("coupling_term", fwd.coupling_term(0)))
-            self.coupling_term = lambda _t: cpl
+            self.coupling_term = lambda _t=0: cpl
         self.linearise_flag = bool(self.linearise(0))
         calc = abs(self.end_time / self.dt) if self.dt else 0.0
         for sh in fwd.physical_domain.get_shape():
