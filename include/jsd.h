@@ -184,6 +184,19 @@ int jsd_dissipation_accumulate(jsd_plan* plan, const void* vel_hat, void* acc, d
  * component (PhysicalField.volume_integral, field.py:1456-1466). */
 int jsd_volume_integral(jsd_plan* plan, const void* f_hat, double* out, void* stream);
 
+/* No counterpart in the reference: jsd_energy for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Needs no plan and allocates nothing: vel_hat
+ * (3 components, the layout above), y_weights (ny definite-integral
+ * weights) and out (1 double) are device pointers; the energy is written
+ * to *out on the stream, without a host copy or synchronisation.  The
+ * 1024 partial sums live in a static 8 KB device array per GPU, shared by
+ * every call on that GPU: calls that overlap on one GPU race.  ny <= 132,
+ * odd nxm, nzm.  The same kernels and grid as jsd_energy: the results are
+ * identical. */
+int jsd_energy_noalloc(const void* vel_hat, int nxm, int ny, int nzm, int px, int pz, double lx, double lz,
+                       const double* y_weights, double* out, void* stream);
+
 /* out = scale * dE/d(vel_hat) on the real-pair view, E the energy of
  * jsd_energy (field.py:607-611). */
 int jsd_energy_grad(jsd_plan* plan, const void* vel_hat, void* out, double scale, void* stream);

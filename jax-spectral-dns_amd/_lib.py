@@ -78,6 +78,7 @@ EXPORTS = (
     "jsd_laplacian",
     "jsd_dissipation_accumulate",
     "jsd_volume_integral",
+    "jsd_energy_noalloc",
     "jsd_batch_create",
     "jsd_batch_destroy",
     "jsd_batch_step",
@@ -113,6 +114,8 @@ def load() -> ctypes.CDLL:
     lib.jsd_laplacian.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, vp]
     lib.jsd_dissipation_accumulate.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_volume_integral.argtypes = [vp, vp, _dp, vp]
+    lib.jsd_energy_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_double, vp, vp, vp]
     lib.jsd_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, vp,
                                      ctypes.POINTER(vp)]
     lib.jsd_batch_destroy.argtypes = [vp]

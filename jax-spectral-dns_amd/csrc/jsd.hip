@@ -3009,6 +3009,61 @@ extern "C" int jsd_energy(jsd_plan* p, const void* vel_hat, double* out, void* s
   return JSD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free energy: the partial sums live in a file-scope device array
+// ---------------------------------------------------------------------------
+// JSD_SCRATCH_DOUBLES partial sums (8 KB), one per workgroup of
+// k_energy_partial: the same grid as jsd_energy, so the two agree bit for
+// bit.  Every workgroup writes its entry before k_energy_final reads it
+// (stream order), so nothing needs clearing; calls that overlap on one GPU
+// share the array and race.
+#define JSD_SCRATCH_DOUBLES 1024
+#define JSD_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[JSD_SCRATCH_DOUBLES];
+static double* g_scratch_dev[JSD_MAX_DEVICES];  // device address of scratch, per device
+
+// The device address of scratch on the current device, looked up once per
+// device (the host-side name is a stand-in, never passed to a kernel).
+static int scratch_on_current_device(double** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= JSD_MAX_DEVICES) JSD_FAIL(JSD_ERR_ARG, "device %d beyond the %d the library tracks", dev,
+                                                  JSD_MAX_DEVICES);
+  double* ptr = __atomic_load_n(&g_scratch_dev[dev], __ATOMIC_ACQUIRE);
+  if (!ptr) {
+    void* a = nullptr;
+    HIPCHK(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    if (!a) JSD_FAIL(JSD_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    ptr = (double*)a;
+    __atomic_store_n(&g_scratch_dev[dev], ptr, __ATOMIC_RELEASE);
+  }
+  *out = ptr;
+  return JSD_OK;
+}
+
+extern "C" int jsd_energy_noalloc(const void* vel_hat, int nxm, int ny, int nzm, int px, int pz, double lx,
+                                  double lz, const double* y_weights, double* out, void* stream) {
+  if (!vel_hat || !y_weights || !out) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (nxm < 1 || nzm < 1 || (nxm & 1) == 0 || (nzm & 1) == 0)
+    JSD_FAIL(JSD_ERR_ARG, "stored mode counts must be odd (got %d, %d)", nxm, nzm);
+  if (ny < 3 || ny > JSD_NYMAX) JSD_FAIL(JSD_ERR_ARG, "ny = %d outside 3..%d", ny, JSD_NYMAX);
+  if (px < 1 || pz < 1 || !(lx > 0.0) || !(lz > 0.0)) JSD_FAIL(JSD_ERR_ARG, "invalid physical grid or box");
+  const size_t fs = (size_t)nxm * ny * nzm;
+  if (3 * fs > ((size_t)1 << 40)) JSD_FAIL(JSD_ERR_ARG, "field of %zu elements beyond the supported size", fs);
+  double* part = nullptr;
+  int rc = scratch_on_current_device(&part);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int npartial = JSD_SCRATCH_DOUBLES;
+  const double sl = (lx / (2 * M_PI)) * (lz / (2 * M_PI));
+  const double scale = 0.25 * sl * sl / ((double)px * (double)pz);
+  hipLaunchKernelGGL(k_energy_partial, dim3(npartial), dim3(256), 0, st, (const double2*)vel_hat, (long)fs,
+                     y_weights, nxm, ny, nzm, part);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, part, npartial, scale, out);
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
 extern "C" int jsd_ydiff(jsd_plan* p, const void* in, void* out, int ncomp, int order, void* stream) {
   if (!p || !in || !out || ncomp < 1 || (order != 1 && order != 2)) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   return launch_ygemm_plain(p, (const double2*)in, (double2*)out, ncomp, order, (hipStream_t)stream);
