@@ -137,18 +137,12 @@ def batch_bench(nsteps: int) -> dict:
     from jax_spectral_dns_amd.plan import StepPlan
     from jax_spectral_dns_amd.vjp import gain_and_grad
 
-    rng = np.random.default_rng(0)
-    shapes = [(16, 16), (32, 16), (48, 32)]
+    from jax_spectral_dns_amd.batch import seeded_linearised_runs
+
     plans, u0 = [], []
     t0 = time.time()
-    for r in range(64):
-        re = rng.uniform(500.0, 6000.0)
-        alpha = rng.uniform(0.5, 2.0)
-        beta = rng.uniform(0.0, 2.0)
-        ny = int(rng.choice([65, 97, 129]))
-        nx, nz = shapes[int(rng.integers(0, 3))]
-        lz = 2 * np.pi / beta if beta >= 1e-3 else 1e-3
-        g = DM.make_grid((nx, ny, nz), 2 * np.pi / alpha, lz)
+    for r, (shape, re, lx, lz) in enumerate(seeded_linearised_runs(64, 0)):
+        g = DM.make_grid(shape, lx, lz)
         plans.append(StepPlan(g, re, 0.01, KIND_PERTURBATION, linearise=True))
         plans[-1].ops = None  # the host copies of the tables are not needed once on the device
         u0.append(torch.from_numpy(initial.perturbation(g, seed=r, energy=1e-6)).cuda())

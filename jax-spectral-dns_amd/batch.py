@@ -25,6 +25,28 @@ from .plan import StepPlan, _dev, _stream
 STAGES = ("ygemm", "nl_lin", "solve", "solve_T", "hv_T", "nl_lin_T", "acc")
 
 
+def seeded_linearised_runs(n: int = 64, seed: int = 0) -> List[Tuple[Tuple[int, int, int], float, float, float]]:
+    """The batch workload of uneven linearised runs, drawn from one seed:
+    (requested shape, Re, Lx, Lz) with Re ~ U[500, 6000], alpha ~ U[0.5, 2],
+    beta ~ U[0, 2], Ny in {65, 97, 129}, (Nx, Nz) in {(16, 16), (32, 16),
+    (48, 32)}, Lx = 2 pi / alpha and Lz = 2 pi / beta (1e-3 for
+    beta < 1e-3, the reference's quasi-2D box, examples.py:1303)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    shapes = [(16, 16), (32, 16), (48, 32)]
+    out = []
+    for _ in range(n):
+        re = rng.uniform(500.0, 6000.0)
+        alpha = rng.uniform(0.5, 2.0)
+        beta = rng.uniform(0.0, 2.0)
+        ny = int(rng.choice([65, 97, 129]))
+        nx, nz = shapes[int(rng.integers(0, 3))]
+        lz = 2 * np.pi / beta if beta >= 1e-3 else 1e-3
+        out.append(((nx, ny, nz), float(re), float(2 * np.pi / alpha), float(lz)))
+    return out
+
+
 def check_distinct(plans: Sequence[object], states: Sequence[torch.Tensor],
                    adjoints: Optional[Sequence[torch.Tensor]] = None) -> None:
     """Every run of a batch writes its own plan's workspaces inside each stage

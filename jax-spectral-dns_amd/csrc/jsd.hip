@@ -1880,6 +1880,8 @@ static int eigen_tables(jsd_plan* p, const jsd_plan_desc* d, int N, size_t nk, c
   }
   const size_t nn = (size_t)N * N;
   for (size_t k = 0; k < nk; ++k) {
+    // the checked entries: every 13th |k| and the last (the host may fill
+    // only these in eigen form, domain_math.check_indices)
     if (k % 13 != 0 && k != nk - 1) continue;
     const int a = (int)(k / p->Kb), b = (int)(k % p->Kb);
     const double k2 = d->kx[a] * d->kx[a] + d->kz[b] * d->kz[b];

@@ -186,6 +186,7 @@ class OperatorTables:
     rhs_ns: np.ndarray  # (3, 2Ny, 2Ny)
     lhs_inv_ns: np.ndarray  # (3, 2Ny, 2Ny)
     y_eig: np.ndarray  # interior_eigen(D^2), flat
+    full: bool = True  # False: lhs_inv / poisson hold only the |k| entries of check_indices
 
 
 def parity_blocks(A: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
@@ -266,7 +267,24 @@ def interior_eigen(D2: np.ndarray) -> np.ndarray:
     return np.concatenate([Ee.ravel(), Eo.ravel(), Xe.ravel(), Xo.ravel(), le, lo])
 
 
-def build_operators(g: Grid, re_tau: float, dt: float) -> OperatorTables:
+def check_indices(nk: int) -> np.ndarray:
+    """The |k| table entries at which plan creation checks the eigen form of
+    the solve against the inverse tables (every 13th and the last)."""
+    k = np.arange(nk)
+    return k[(k % 13 == 0) | (k == nk - 1)]
+
+
+def build_operators(g: Grid, re_tau: float, dt: float, full: bool = True) -> OperatorTables:
+    """The per-|k| operator tables of navier_stokes.py:920-1022 and
+    domain.py:857-889.  full = False (the eigen-form solve): the Helmholtz
+    and Poisson inverses are formed only at check_indices (the plan's
+    cross-check of its eigen factorisation), the other entries stay zero,
+    and the wall-correction solution v_a is solved in the eigen coordinates
+    of D^2's even / odd interior halves, where both of its implicit problems
+    are diagonal (the same values to rounding, O(Ny^2) per |k| instead of
+    two Ny x Ny inversions)."""
+    if not full:
+        return _build_operators_sampled(g, re_tau, dt)
     n = g.ny
     ka, kb = g.ka, g.kb
     D2 = np.linalg.matrix_power(g.D, 2)
@@ -328,6 +346,101 @@ def build_operators(g: Grid, re_tau: float, dt: float) -> OperatorTables:
         lhs_inv_ns=lhs_inv_ns,
         y_eig=interior_eigen(D2),
     )
+
+
+def _build_operators_sampled(g: Grid, re_tau: float, dt: float) -> OperatorTables:
+    n = g.ny
+    ka, kb = g.ka, g.kb
+    D2 = np.linalg.matrix_power(g.D, 2)
+    Ly = 1 / re_tau * D2
+    I = np.eye(n)
+    kxa = g.kx[:ka]
+    kzb = g.kz[:kb]
+    k2 = (kxa[:, None] ** 2 + kzb[None, :] ** 2).reshape(-1)
+    nk = ka * kb
+    ck = check_indices(nk)
+    L = Ly[None, :, :] + I[None, :, :] * (-(k2))[:, None, None] / re_tau
+    lhs_inv = np.zeros((3, nk, n, n))  # untouched pages: no memory beyond the checked entries
+    poisson = np.zeros((nk, n, n))
+    v_a = np.empty((3, nk, n))
+    infl = np.empty((3, nk, 4))
+    rhs_base = np.empty((3, n, n))
+    eye_bc = np.eye(n)
+    eye_bc[0, 0] = 0.0
+    eye_bc[-1, -1] = 0.0
+    D2h = hom_dir(D2)
+    P = (-(k2))[:, None, None] * eye_bc[None] + D2h[None]
+    poisson[ck] = np.linalg.inv(P[ck])
+    y_eig = interior_eigen(D2)
+    Ee, Eo, Xe, Xo, le, lo = _split_eigen(y_eig, n)
+    ne, no = len(le), len(lo)
+    # eigen coordinates of the interior of D^2's wall column (the right-hand
+    # side of the inhomogeneous Helmholtz problem, times c)
+    b0 = D2[1 : n - 1, 0]
+    h = np.arange(no)
+    be = 0.5 * (b0[:ne] + b0[::-1][:ne])
+    bo = 0.5 * (b0[h] - b0[n - 3 - h])
+    ze, zo = Xe @ be, Xo @ bo
+    for s in range(3):
+        A = I[None] - RK_BETA[s] * dt * L
+        lhs = A[ck].copy()
+        lhs[:, 0, :] = 0.0
+        lhs[:, :, 0] = 0.0
+        lhs[:, -1, :] = 0.0
+        lhs[:, :, -1] = 0.0
+        lhs[:, 0, 0] = 1.0
+        lhs[:, -1, -1] = 1.0
+        lhs_inv[s][ck] = np.linalg.inv(lhs)
+        # phi_a = inv(L_inh) e_0: phi_0 = 1, phi_{N-1} = 0 and on the interior
+        # ((1 + c k^2) I - c D^2) phi = c D^2[:, 0]; v_a = (D^2 - k^2)^-1 phi on
+        # the interior: both diagonal in the eigen coordinates of D^2's halves
+        c = RK_BETA[s] * dt / re_tau
+        de = ((1.0 + c * k2)[None, :] - c * le[:, None]) * (le[:, None] - k2[None, :])
+        do = ((1.0 + c * k2)[None, :] - c * lo[:, None]) * (lo[:, None] - k2[None, :])
+        ve = Ee @ (c * ze[:, None] / de)  # (ne, nk)
+        vo = Eo @ (c * zo[:, None] / do)  # (no, nk)
+        vint = np.empty((n - 2, nk))
+        vint[:ne] = ve
+        vint[h] += vo
+        vint[n - 3 - h] = ve[h] - vo
+        va = np.zeros((nk, n))
+        va[:, 1 : n - 1] = vint.T
+        vb = va[:, ::-1]
+        dva = va @ g.D.T
+        dvb = vb @ g.D.T
+        M = np.stack([np.stack([dva[:, 0], dvb[:, 0]], -1), np.stack([dva[:, -1], dvb[:, -1]], -1)], -2)
+        infl[s] = np.linalg.inv(M).reshape(nk, 4)
+        v_a[s] = va
+        rhs_base[s] = hom_dir(I + RK_ALPHA[s] * dt * Ly)
+    Z = np.zeros((n, n))
+    L_ns = 1 / re_tau * np.block([[D2h, Z], [Z, D2h]])
+    I2 = np.eye(2 * n)
+    rhs_ns = np.stack([hom_dir(I2 + RK_ALPHA[s] * dt * L_ns) for s in range(3)])
+    lhs_inv_ns = np.stack([np.linalg.inv(hom_dir(I2 - RK_BETA[s] * dt * L_ns)) for s in range(3)])
+    return OperatorTables(
+        lhs_inv=lhs_inv.reshape(3, ka, kb, n, n),
+        poisson=poisson.reshape(ka, kb, n, n),
+        v_a=v_a.reshape(3, ka, kb, n),
+        infl_inv=infl.reshape(3, ka, kb, 4),
+        rhs_base=rhs_base,
+        rhs_ns=rhs_ns,
+        lhs_inv_ns=lhs_inv_ns,
+        y_eig=y_eig,
+        full=False,
+    )
+
+
+def _split_eigen(y_eig: np.ndarray, n: int):
+    """(E_e, E_o, E_e^-1, E_o^-1, lam_e, lam_o) of interior_eigen's flat layout."""
+    ne, no = n // 2 - 1 + (n & 1), n // 2 - 1
+    o = 0
+    out = []
+    for r in (ne, no, ne, no):
+        out.append(y_eig[o : o + r * r].reshape(r, r))
+        o += r * r
+    out.append(y_eig[o : o + ne])
+    out.append(y_eig[o + ne : o + ne + no])
+    return out
 
 
 def dit_tables(p: int, inverse: bool) -> Tuple[np.ndarray, np.ndarray, List[int]]:

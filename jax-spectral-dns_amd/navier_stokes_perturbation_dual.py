@@ -93,8 +93,14 @@ class NavierStokesVelVortPerturbationDual:
         operator tables (navier_stokes_perturbation_dual.py:470-471)."""
         key = (bool(self.linearise(t)), bool(self.coupling_term(t)))
         if key not in self._plans:
+            # kept on the forward equation, so the duals an optimisation
+            # driver builds from it every iteration share the plans
+            cache = self.forward_equation.__dict__.setdefault("_dual_plan_cache", {})
+            if key in cache:
+                self._plans[key] = cache[key]
+                return cache[key]
             fplan = self.forward_equation.plan()
-            self._plans[key] = StepPlan(
+            self._plans[key] = cache[key] = StepPlan(
                 fplan.grid,
                 fplan.re_tau,
                 fplan.dt,

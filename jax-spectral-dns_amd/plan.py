@@ -61,7 +61,11 @@ class StepPlan:
         self.linearise = bool(linearise)
         self.re_tau = re_tau
         self.dt = dt
-        self.ops = ops if ops is not None else build_operators(grid, re_tau, dt)
+        # in eigen form the per-|k| inverse tables serve only the plan's
+        # cross-check of the factorisation: build them at the checked |k| only
+        self.ops = ops if ops is not None else build_operators(grid, re_tau, dt, full=not eigen_solve)
+        if not eigen_solve and not self.ops.full:
+            raise ValueError("the table-form solve needs full operator tables (build_operators(..., full=True))")
         g, o = grid, self.ops
         self._keep = []
 

@@ -7,6 +7,10 @@ cfg0: linearised perturbation about U = 1 - y^2, Re = 3000, Lx = Lz = 2 pi,
 24 x 65 x 4 requested modes (25 x 65 x 5 stored), dt = 0.01, 1500 steps;
 initial field: seeded generator, seed 0, energy 1e-6.  Stores the initial
 state, the states after 1, 10 and 1500 steps and the energy history.
+
+    python tests/golden/make_golden.py cfg1
+
+cfg1: see cfg1() (200 nonlinear steps at 128 x 129 x 128; an hour of CPU).
 """
 
 import os
@@ -48,6 +52,38 @@ def cfg0() -> None:
         params=np.array([re, dt, n, 1e-6, 0.0]),
     )
     print("cfg0: n=%d gain E(T)/E(0) = %.12f" % (n, energy[-1] / energy[0]))
+
+
+def cfg1() -> None:
+    """cfg1: plain nonlinear NS, centreline Re = 5000, Lx = 4 pi, Lz = 2 pi,
+    128 x 129 x 128 requested modes (129 x 129 x 129 stored), dt = 0.005,
+    dPdx = -2/Re, 200 steps; initial field: Poiseuille flow plus the seeded
+    perturbation (seed 0) at 5 % of its energy.  Stores the energy after every
+    step and the (0,0) mean-flow lines of u and w after 1, 10, 100 and 200
+    steps (the full states are 103 MB each)."""
+    torch.set_num_threads(int(os.environ.get("GOLDEN_THREADS", "8")))
+    shape, re, dt, n = (128, 129, 128), 5000.0, 0.005, 200
+    lx, lz = 4 * np.pi, 2 * np.pi
+    g = DM.make_grid(shape, lx, lz)
+    d = O.make_domain(shape, lx, lz)
+    base = initial.laminar(g)
+    u0 = torch.from_numpy(base + initial.perturbation(g, seed=0, energy=0.05 * initial.spectral_energy(g, base)))
+    sp = O.make_ns(d, re, dt, -2.0 / re)
+    u = u0.clone()
+    energy = [float(O.energy(d, u))]
+    lines = {}
+    for t in range(n):
+        u = O.rk_step(sp, u, t)
+        energy.append(float(O.energy(d, u)))
+        if t + 1 in (1, 10, 100, 200):
+            lines["mean%d" % (t + 1)] = torch.stack([u[0, 0, :, 0], u[2, 0, :, 0]]).numpy().copy()
+        print("cfg1 step %d energy %.16e" % (t + 1, energy[-1]), flush=True)
+    np.savez_compressed(
+        os.path.join(HERE, "cfg1_ns_200_energy.npz"),
+        energy=np.array(energy),
+        params=np.array([re, dt, n, 0.05, 0.0]),
+        **lines,
+    )
 
 
 if __name__ == "__main__":

@@ -213,3 +213,26 @@ def test_parameterisation_matches_oracle():
     a = vort_yvel_to_vel(g, vort, vy, ux, wz)
     b = O.vort_yvel_to_vel(d, vort, vy, ux, wz)
     assert float((a - b).abs().max() / b.abs().max()) < 1e-13
+
+
+def test_sampled_operator_tables_match_full_ones():
+    """build_operators(full=False) (the eigen-form plans' build) equals the
+    full build at the checked |k| entries, and its wall-correction tables
+    (from LU solves) equal the full build's to rounding."""
+    g = DM.make_grid((24, 33, 16), 2 * np.pi, np.pi)
+    full = DM.build_operators(g, 2000.0, 0.01)
+    samp = DM.build_operators(g, 2000.0, 0.01, full=False)
+    assert full.full and not samp.full
+    nk = g.ka * g.kb
+    ck = DM.check_indices(nk)
+    assert ck[0] == 0 and ck[-1] == nk - 1 and all(k % 13 == 0 or k == nk - 1 for k in ck)
+    li_f = full.lhs_inv.reshape(3, nk, g.ny, g.ny)
+    li_s = samp.lhs_inv.reshape(3, nk, g.ny, g.ny)
+    assert np.array_equal(li_f[:, ck], li_s[:, ck])
+    po_f = full.poisson.reshape(nk, g.ny, g.ny)
+    po_s = samp.poisson.reshape(nk, g.ny, g.ny)
+    assert np.array_equal(po_f[ck], po_s[ck])
+    assert np.abs(samp.v_a - full.v_a).max() < 1e-13 * np.abs(full.v_a).max()
+    assert np.abs(samp.infl_inv - full.infl_inv).max() < 1e-10 * np.abs(full.infl_inv).max()
+    for name in ("rhs_base", "rhs_ns", "lhs_inv_ns", "y_eig"):
+        assert np.array_equal(getattr(samp, name), getattr(full, name)), name

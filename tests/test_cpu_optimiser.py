@@ -20,7 +20,9 @@ def _solver(x0, value_and_grad, step_size=1e-3):
     ow.step_size = step_size
     ow.x = x0.clone()
     ow.evaluations = 0
+    ow.precondition = False  # the plain method (the preconditioner is the energy metric of the flow problem)
     ow.f, ow.g = value_and_grad(ow.x)
+    ow.y = ow.g
     ow.d = -ow.g
     ow.value = -ow.f
     ow.values, ow.steps, ow.wolfe, ow.i = [ow.value], [], [], 0

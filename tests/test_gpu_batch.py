@@ -155,3 +155,47 @@ def test_library_rejects_repeated_plans_in_a_batch():
         h = vp()
         rc = lib.jsd_batch_create(hp, sp, None, 2, st, ctypes.byref(h))
         assert rc == -1 and not h.value
+
+
+def test_cfg3_seeded_64_run_batch():
+    """The batch workload itself: the 64 runs drawn from seed 0 (Ny up to 129,
+    (Nx, Nz) up to (48, 32)), 20 batched steps against the same runs stepped
+    one at a time (<= 1e-12), and one Ny = 129, 48 x 32 run of the draw
+    against the oracle for 2 steps (<= 2e-11)."""
+    from jax_spectral_dns_amd import initial
+    from jax_spectral_dns_amd.batch import seeded_linearised_runs
+
+    runs = seeded_linearised_runs(64, 0)
+    assert {r[0][1] for r in runs} == {65, 97, 129}
+    assert {(r[0][0], r[0][2]) for r in runs} == {(16, 16), (32, 16), (48, 32)}
+    plans, u0 = [], []
+    for r, (shape, re, lx, lz) in enumerate(runs):
+        g = DM.make_grid(shape, lx, lz)
+        plans.append(StepPlan(g, re, 0.01, KIND_PERTURBATION, linearise=True))
+        plans[-1].ops = None
+        u0.append(torch.from_numpy(initial.perturbation(g, seed=r, energy=1e-6)).to(DEV))
+    ind = [u.clone() for u in u0]
+    for p, u in zip(plans, ind):
+        p.step(u, 0.0, 20)
+    bat = [u.clone() for u in u0]
+    batch = BatchPlan(plans, bat)
+    batch.step(20)
+    torch.cuda.synchronize()
+    worst = max(rel(a, b) for a, b in zip(bat, ind))
+    print("64-run batch vs one at a time after 20 steps: %.3e" % worst)
+    assert worst <= 1e-12
+    batch.close()
+    k = next(i for i, r in enumerate(runs) if r[0] == (48, 129, 32))
+    shape, re, lx, lz = runs[k]
+    d = O.make_domain(shape, lx, lz)
+    sp = O.make_perturbation(d, re, 0.01, True)
+    ref = u0[k].cpu()
+    for n in range(2):
+        ref = O.rk_step(sp, ref, n)
+    b2 = [u.clone() for u in u0]
+    batch = BatchPlan(plans, b2)
+    batch.step(2)
+    err = rel(b2[k], ref)
+    print("run %d %s vs oracle after 2 steps: %.3e" % (k, shape, err))
+    assert err < 2e-11
+    batch.close()

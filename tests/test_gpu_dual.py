@@ -130,32 +130,32 @@ def test_conjugate_gradient_descent_increases_gain():
     assert abs(e - solver.e_0) <= 1e-12 * solver.e_0
 
 
-def test_optimiser_wrapper_reaches_the_optimal_growth_of_the_box():
+def test_optimiser_wrapper_reaches_the_optimal_transient_growth():
     """OptimiserWrapper (gradient_descent_solver.py:862-944: the gain and its
-    gradient handed to nonlinear CG with a strong-Wolfe zoom line search)
-    from a seeded random start at Re 1000, T = 5, on a quasi-2D box
-    (alpha in 0..4, Lz = 1e-3 so every kz != 0 mode decays at once).  Its
-    gradient matches centred differences, the gain never decreases, more
-    than triples in 40 iterations, and stays below the largest optimal
-    transient growth over the box's alpha from the linear stability
-    calculation (the upper bound any initial condition obeys; G = 7.91 at
-    alpha = 2).  (An oblique wave whose |kz| is the
-    largest stored one is distorted by the reference's +-K storage
-    convention, which is why the pin uses a box where those modes are
-    inert.)"""
+    gradient handed to nonlinear CG with a strong-Wolfe zoom line search) on
+    the reference's own optimisation case (run_optimisation_transient_growth_y_profile,
+    examples.py:1634-1780; README.md:145-148): Re 3000, alpha = 1, T = 15,
+    dt = 0.01, Ny = 90, starting from the stability optimum truncated to 20
+    modes, whose gain the reference reports as 1.24.  The reference reaches
+    19.29 in 20 iterations; the preconditioned CG here reaches 0.95 G(15)
+    after one iteration and G(15) = 20.3625 (the stability calculation with
+    90 modes) to 0.1 % within five.  The gradient matches centred
+    differences and the initial energy is kept."""
+    from jax_spectral_dns_amd.domain import PhysicalDomain
     from jax_spectral_dns_amd.gradient_descent_solver import OptimiserWrapper
     from jax_spectral_dns_amd.linear_stability_calculation import LinearStabilityCalculation
 
-    shape, lx, lz, re, dt, T = (8, 49, 4), 2 * np.pi, 1e-3, 1000.0, 0.01, 5.0
-    g, _ = pair(shape, lx, lz)
-    G_box = max(
-        LinearStabilityCalculation(Re=re, alpha=float(a), beta=0.0, n=g.ny).calculate_transient_growth_max_energy(T, 60)
-        for a in range(1, 5)
-    )
-    u0 = perturbation_field(g, seed=0, energy=1e-6)
-    ow = OptimiserWrapper(_nse(shape, lx, lz, re, dt, T, True, u0), max_iterations=40)
-    # the gradient handed to the line search against centred differences of
-    # the objective along a seeded direction in parameter space
+    re, T, dt = 3000.0, 15.0, 0.01
+    shape = (8, 90, 8)
+    pd = PhysicalDomain.create(shape, (True, False, True), scale_factors=(2 * np.pi, 1.0, 2 * np.pi))
+    lsc = LinearStabilityCalculation(Re=re, alpha=1.0, beta=0.0, n=90)
+    G = lsc.calculate_transient_growth_max_energy(T, 90)
+    assert abs(G - 20.36) < 0.01  # README.md:147-148
+    u0 = lsc.calculate_transient_growth_initial_condition(pd, T, 20, energy=1e-6)
+    vh = VectorField.FromData(FourierField, pd, u0.to("cuda"), "velocity_hat")
+    nse = NavierStokesVelVortPerturbation(vh, Re=re, dt=dt, end_time=T, linearise=True)
+    ow = OptimiserWrapper(nse, max_iterations=5)
+    assert abs(ow.value - 1.24) < 0.005
     gen = torch.Generator().manual_seed(1)
     dirn = (torch.randn(ow.x.shape, dtype=torch.complex128, generator=gen) * float(ow.x.abs().mean())).to("cuda")
     h = 1e-3
@@ -166,10 +166,9 @@ def test_optimiser_wrapper_reaches_the_optimal_growth_of_the_box():
     print("directional derivative: adjoint %.8e, centred differences %.8e" % (an, fd))
     assert abs(an - fd) < 1e-3 * abs(fd)
     vals = ow.optimise()
-    print("gains", ["%.6f" % v for v in vals], "box optimum %.6f" % G_box, "evaluations", ow.evaluations,
-          "wolfe", sum(ow.wolfe), "/", len(ow.wolfe))
+    print("gains", ["%.4f" % v for v in vals], "G(15) %.4f" % G, "evaluations", ow.evaluations)
+    assert vals[1] >= 0.95 * G
     assert all(b >= a * (1 - 1e-12) for a, b in zip(vals, vals[1:]))
-    assert vals[-1] > 3.0 * vals[0]
-    assert max(vals) < 1.002 * G_box
+    assert abs(vals[-1] - G) < 1e-3 * G
     e = ow.plan.energy(ow.current_guess)
     assert abs(e - ow.e_0) <= 1e-12 * ow.e_0

@@ -106,6 +106,44 @@ def test_gain_gradient_against_finite_differences():
     assert worst < 1e-8
 
 
+def test_cfg2_full_horizon_gradient_against_finite_differences():
+    """The differentiation workload at full length: 48 x 97 x 32, Re 3000,
+    dt 0.01, T = 15 (1500 steps), checkpoint every 50; the exact-quadratic
+    centred differences along 16 seeded directions (<= 1e-8)."""
+    shape = (48, 97, 32)
+    g, _ = pair(shape)
+    re, dt, n = 3000.0, 0.01, 1500
+    u0 = perturbation_field(g, seed=0, energy=1e-6).to(DEV)
+    plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=True)
+    J, grad = gain_and_grad(u0, plan, n, checkpoint_every=50)
+
+    def energies(u):
+        v = u.clone()
+        plan.step(v, 0.0, n)
+        return plan.energy(v), plan.energy(u)
+
+    h = 0.5
+    N0, D0 = energies(u0)
+    assert abs(N0 / D0 - J) <= 1e-12 * abs(J)
+    worst = 0.0
+    for k in range(16):
+        rng = np.random.default_rng(1 + k)
+        dirn = torch.from_numpy(rng.standard_normal(u0.shape) + 1j * rng.standard_normal(u0.shape)).to(DEV)
+        dirn[:, :, 0] = 0
+        dirn[:, :, -1] = 0
+        dirn *= torch.linalg.norm(u0) / torch.linalg.norm(dirn)
+        Np, Dp = energies(u0 + h * dirn)
+        Nm, Dm = energies(u0 - h * dirn)
+        dN, dD = (Np - Nm) / (2 * h), (Dp - Dm) / (2 * h)
+        fd = (dN * D0 - N0 * dD) / (D0 * D0)
+        ad = _dot(grad, dirn)
+        e = abs(fd - ad) / max(abs(ad), 1e-300)
+        worst = max(worst, e)
+        print("dir %2d fd %.12e ad %.12e rel %.2e" % (k, fd, ad, e), flush=True)
+    print("J(T = 15) = %.12f" % J)
+    assert worst < 1e-8
+
+
 def test_parameter_gradient_against_oracle_autograd():
     """dJ/d(omega_y, v, u_00, w_00) through vort_yvel_to_vel (the reference's
     optimisation parameters) against autograd of the oracle, <= 1e-9."""
