@@ -80,6 +80,51 @@ def ygemm_flops(g, ncomp: int = 3) -> float:
     return 4.0 * nef * nof * (2 * g.nzm) * g.nxm * ncomp
 
 
+def step_model_bytes(g, n_phys: int = 12) -> float:
+    """The whole step's canonical byte count (each stage reads its inputs
+    once and writes its outputs once): 3 substeps of 19 S + (2 n_phys + 6) R
+    + 2 F, S one complex128 spectral field, R one FP64 field on the
+    reference's 3/2 physical grid, F one family of real per-|k| operators
+    (n_phys = 12 for the nonlinear term: u and its gradient)."""
+    S = 16.0 * g.ny * g.nxm * g.nzm
+    R = 8.0 * g.ny * g.px * g.pz
+    F = 8.0 * g.ny * g.ny * g.ka * g.kb
+    return 3.0 * (19.0 * S + (2 * n_phys + 6) * R + 2.0 * F)
+
+
+def copy_bandwidth_gbs(dev, nbytes: int = 4 << 30, reps: int = 20) -> float:
+    """Device-to-device copy rate of a buffer far larger than the last-level
+    cache (read + write bytes / time, HIP events), the measured counterpart
+    of the 8 TB/s HBM peak."""
+    import torch
+
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def roofline_entry(dominant, mfma_bound, nflops, nbytes, dom_ms, achieved_gbs, traffic) -> dict:
     """The dominant kernel against its binding roofline (FP64 MFMA when its
     flops at peak take longer than its bytes at 8 TB/s, HBM otherwise)."""
@@ -256,7 +301,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-runs figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the nonlinear-gradient figure")
     ap.add_argument("--adjoint-steps", type=int, default=40)
@@ -290,7 +335,7 @@ def main() -> int:
     dpdx = -2.0 / re
     g = DM.make_grid(shape, lx, lz)
     t0 = time.time()
-    ops = DM.build_operators(g, re, dt)
+    ops = DM.build_operators(g, re, dt, full=False)  # eigen-form plan: inverse tables only where checked
     t_pre = time.time() - t0
     plan = StepPlan(g, re, dt, KIND_NS, ops=ops)
     base = initial.laminar(g)
@@ -349,6 +394,16 @@ def main() -> int:
     ygemm_ms = per_class["ygemm"] / max(1, per_class_n["ygemm"])  # warm-up events
     step_ms = 1e3 * elapsed / args.steps
     finite = bool(torch.isfinite(u).all())
+    # whole-step fractions: the canonical byte model, and the HBM traffic the
+    # calibrated counter runs measured per launch times the launches per step
+    b_model = step_model_bytes(g)
+    copy_gbs = copy_bandwidth_gbs(dev)
+    measured_step = None
+    if os.path.exists(tpath):
+        kern = json.load(open(tpath)).get("kernels", {})
+        if all(c in kern for c in CLASSES if per_class_n[c] > 0):
+            measured_step = sum(kern[c]["bytes_per_launch"] * per_class_n[c] / nprof for c in CLASSES if per_class_n[c] > 0)
+    step_s = elapsed / args.steps
 
     out = {
         "metric": METRIC,
@@ -372,7 +427,16 @@ def main() -> int:
             "operator_precompute_s": round(t_pre, 3),
             "state_finite": finite,
         },
-        "roofline": roofline_entry(dominant, mfma_bound, nflops, nbytes, dom_ms, achieved_gbs, traffic),
+        "roofline": dict(
+            roofline_entry(dominant, mfma_bound, nflops, nbytes, dom_ms, achieved_gbs, traffic),
+            step_model_bytes=b_model,
+            step_model_frac=b_model / step_s / (HBM_PEAK_GBS * 1e9),
+            measured_step_bytes=measured_step,
+            measured_traffic_frac=None if measured_step is None else measured_step / step_s / (HBM_PEAK_GBS * 1e9),
+            copy_peak_gbs=copy_gbs,
+            step_model_frac_vs_copy=b_model / step_s / (copy_gbs * 1e9),
+            measured_traffic_frac_vs_copy=None if measured_step is None else measured_step / step_s / (copy_gbs * 1e9),
+        ),
         # every class, bracketed by events during the warm-up steps
         "kernel_ms_per_step": {c: per_class[c] / nprof for c in CLASSES},
         "mfma_ygemm": {
@@ -389,8 +453,7 @@ def main() -> int:
         # bounded CPU sample: one oracle step from u0, also the parity gate
         from oracle import jsd_oracle as O
 
-        ncores = len(os.sched_getaffinity(0))
-        threads = min(16, ncores)
+        threads = len(os.sched_getaffinity(0))
         torch.set_num_threads(threads)
         d = O.make_domain(shape, lx, lz)
         sp = O.make_ns(d, re, dt, dpdx)
@@ -406,6 +469,7 @@ def main() -> int:
             "value": args.cpu_steps / t_cpu,
             "unit": "steps/s",
             "cores": threads,
+            "cpu_model": cpu_model(),
             "kind": "port",
             "sample": "%d RK3 step(s) of the same 128x129x128 workload by the oracle (torch FP64 CPU "
             "restatement of the reference step, real per-|k| operator tables), operator build excluded"

@@ -216,8 +216,14 @@ int jsd_ydiff(jsd_plan* plan, const void* in, void* out, int ncomp, int order, v
 int jsd_profile(jsd_plan* plan, int enable, double* ms, long* count);
 
 /* ---- batches of independent runs ------------------------------------------
- * Many linearised runs (JSD_KIND_PERTURBATION or JSD_KIND_DUAL plans with
- * linearise = 1, uneven shapes, Re, dt and box sizes allowed) advanced with
+ * Two kinds of batch.  Nonlinear perturbation runs (JSD_KIND_PERTURBATION
+ * plans with linearise = 0, coupling term on; with adjoint buffers, plans
+ * created with adjoint = 1) step concurrently: every run has a stream of its
+ * own, forked from the caller's stream and joined back to it, so the stage
+ * kernels of different runs overlap on the GPU; their adjoint takes each
+ * run's forward state (jsd_batch_step_vjp_state).  Many linearised runs
+ * (JSD_KIND_PERTURBATION or JSD_KIND_DUAL plans with
+ * linearise = 1, uneven shapes, Re, dt and box sizes allowed) are advanced with
  * one launch per stage for all of them: each stage kernel walks the
  * concatenated per-run grids with a persistent grid whose size is picked by
  * timing candidates at creation (the autotuner), and one RK3 step (forward
@@ -240,11 +246,18 @@ int jsd_batch_create(jsd_plan* const* plans, void* const* states, void* const* a
 int jsd_batch_destroy(jsd_batch* batch);
 /* nsteps RK3 steps of every run (as jsd_step with dPdx = 0 on each). */
 int jsd_batch_step(jsd_batch* batch, int nsteps, void* stream);
-/* nsteps adjoint steps of every run (as jsd_step_vjp on each adjoint). */
+/* nsteps adjoint steps of every run (as jsd_step_vjp on each adjoint);
+ * linearised batches only. */
 int jsd_batch_step_vjp(jsd_batch* batch, int nsteps, void* stream);
+/* One adjoint step of every run, as jsd_step_vjp_state(plan_r, states[r],
+ * adjoint_r, 0) (states[r]: device pointer to run r's forward state at the
+ * start of the step); on a linearised batch the states are not read and
+ * this is jsd_batch_step_vjp with nsteps = 1. */
+int jsd_batch_step_vjp_state(jsd_batch* batch, const void* const* states, void* stream);
 /* Autotuned persistent grid sizes of the 7 stage kernels (forward y-GEMM,
  * linearised term, solve; adjoint solve, h_v/h_g transpose, linearised term
- * transpose, D^T accumulation) and their total logical block counts. */
+ * transpose, D^T accumulation) and their total logical block counts (all 0
+ * for a batch of nonlinear runs, which launches each run's own kernels). */
 int jsd_batch_info(jsd_batch* batch, int* grid, int* blocks);
 
 const char* jsd_last_error(void);

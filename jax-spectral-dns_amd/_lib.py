@@ -83,6 +83,7 @@ EXPORTS = (
     "jsd_batch_destroy",
     "jsd_batch_step",
     "jsd_batch_step_vjp",
+    "jsd_batch_step_vjp_state",
     "jsd_batch_info",
     "jsd_last_error",
     "jsd_version",
@@ -121,6 +122,7 @@ def load() -> ctypes.CDLL:
     lib.jsd_batch_destroy.argtypes = [vp]
     lib.jsd_batch_step.argtypes = [vp, ctypes.c_int, vp]
     lib.jsd_batch_step_vjp.argtypes = [vp, ctypes.c_int, vp]
+    lib.jsd_batch_step_vjp_state.argtypes = [vp, ctypes.POINTER(vp), vp]
     lib.jsd_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.jsd_last_error.restype = ctypes.c_char_p
     lib.jsd_version.restype = ctypes.c_int

@@ -1,4 +1,4 @@
-"""Batches of independent linearised runs on one GPU.
+"""Batches of independent runs on one GPU.
 
 ``BatchPlan`` binds a list of ``StepPlan`` objects (linearised perturbation
 or dual plans of any shape, Re, dt and box) to their device state buffers and
@@ -6,7 +6,10 @@ advances all of them with one launch per stage (``jsd_batch_step``): the
 stage kernels walk the concatenated per-run grids with an autotuned
 persistent grid and one RK3 step is replayed from a captured HIP graph.  With
 adjoint buffers it also runs the batched discrete adjoint
-(``jsd_batch_step_vjp``).  The reference advances such runs one at a time
+(``jsd_batch_step_vjp``).  Nonlinear perturbation plans form a batch whose
+runs step concurrently, one stream each (``jsd_batch_step``), with the
+adjoint taken about each run's forward state (``jsd_batch_step_vjp_state``).
+The reference advances such runs one at a time
 (a loop of perform_time_step, navier_stokes.py:1778-1813, per run);
 ``batch_gain_and_grad`` is the batched form of ``vjp.gain_and_grad`` for the
 gradient evaluations of an optimisation driver.
@@ -92,6 +95,17 @@ class BatchPlan:
     def vjp(self, nsteps: int = 1) -> None:
         _lib.check(_lib.load().jsd_batch_step_vjp(self._h, int(nsteps), _stream()))
 
+    def vjp_state(self, states: Sequence[torch.Tensor]) -> None:
+        """One adjoint step of every run about its forward state at the start
+        of the step (nonlinear runs; linearised batches ignore the states)."""
+        if len(states) != len(self.plans):
+            raise _lib.JsdError("one forward state per run")
+        for p, st in zip(self.plans, states):
+            p._check_state(st)
+        vp = ctypes.c_void_p
+        arr = (vp * len(states))(*[_dev(st) for st in states])
+        _lib.check(_lib.load().jsd_batch_step_vjp_state(self._h, arr, _stream()))
+
     def info(self) -> dict:
         g = (ctypes.c_int * len(STAGES))()
         b = (ctypes.c_int * len(STAGES))()
@@ -117,28 +131,66 @@ def _energy_grad(plan: StepPlan, u: torch.Tensor, scale: float) -> torch.Tensor:
 
 
 class BatchGradient:
-    """J_r = E(u_T)/E(u_0) and dJ_r/du_0 for every run of a batch of linearised
-    perturbation plans: one batched forward of nsteps, the energies, one
-    batched adjoint of nsteps (the linearised step's transpose needs no
-    forward state, so nothing is stored or recomputed)."""
+    """J_r = E(u_T)/E(u_0) and dJ_r/du_0 for every run of a batch.
 
-    def __init__(self, plans: Sequence[StepPlan]):
+    Linearised perturbation plans: one batched forward of nsteps, the
+    energies, one batched adjoint of nsteps (the linearised step's transpose
+    needs no forward state, so nothing is stored or recomputed).
+
+    Nonlinear perturbation plans (created with adjoint = True): the batched
+    forward keeps a checkpoint of every run every ``checkpoint_every``
+    steps; the adjoint walks the chunks backwards, recomputes each chunk's
+    step-start states of all runs with batched steps and takes one batched
+    adjoint step per forward step about them (the reference's checkpoint /
+    recompute scheme, navier_stokes_perturbation_dual.py:730-757, 990-1035,
+    applied to the discrete adjoint)."""
+
+    def __init__(self, plans: Sequence[StepPlan], checkpoint_every: int = 100):
         self.plans = list(plans)
         dev = torch.device("cuda")
         self.u = [torch.zeros(p.shape, dtype=torch.complex128, device=dev) for p in self.plans]
         self.g = [torch.zeros_like(u) for u in self.u]
         self.batch = BatchPlan(self.plans, self.u, self.g)
+        self.nonlinear = any(p.adjoint_needs_state for p in self.plans)
+        self.checkpoint_every = max(1, int(checkpoint_every))
+
+    def _nonlinear_backward(self, u0s: Sequence[torch.Tensor], nsteps: int, ckpts: List[List[torch.Tensor]]) -> None:
+        k = self.checkpoint_every
+        starts = list(range(0, nsteps, k))
+        for ci in reversed(range(len(starts))):
+            c = min(k, nsteps - starts[ci])
+            for u, ck in zip(self.u, ckpts[ci]):
+                u.copy_(ck)
+            hist = [[u.clone()] for u in self.u]
+            for _ in range(c - 1):
+                self.batch.step(1)
+                for h, u in zip(hist, self.u):
+                    h.append(u.clone())
+            for j in reversed(range(c)):
+                self.batch.vjp_state([h[j] for h in hist])
 
     def __call__(self, u0s: Sequence[torch.Tensor], nsteps: int) -> Tuple[List[float], List[torch.Tensor]]:
         for u, u0 in zip(self.u, u0s):
             u.copy_(u0)
-        self.batch.step(nsteps)
+        ckpts: List[List[torch.Tensor]] = []
+        if self.nonlinear:
+            done = 0
+            while done < nsteps:
+                ckpts.append([u.clone() for u in self.u])
+                c = min(self.checkpoint_every, nsteps - done)
+                self.batch.step(c)
+                done += c
+        else:
+            self.batch.step(nsteps)
         e0 = [p.energy(u0) for p, u0 in zip(self.plans, u0s)]
         eT = [p.energy(u) for p, u in zip(self.plans, self.u)]
         J = [a / b for a, b in zip(eT, e0)]
         for p, u, g, E0 in zip(self.plans, self.u, self.g, e0):
             g.copy_(_energy_grad(p, u, 1.0 / E0))
-        self.batch.vjp(nsteps)
+        if self.nonlinear:
+            self._nonlinear_backward(u0s, nsteps, ckpts)
+        else:
+            self.batch.vjp(nsteps)
         grads = []
         for p, u0, g, j, E0 in zip(self.plans, u0s, self.g, J, e0):
             grads.append(g - _energy_grad(p, u0, j / E0))
@@ -149,9 +201,9 @@ class BatchGradient:
 
 
 def batch_gain_and_grad(
-    plans: Sequence[StepPlan], u0s: Sequence[torch.Tensor], nsteps: int
+    plans: Sequence[StepPlan], u0s: Sequence[torch.Tensor], nsteps: int, checkpoint_every: int = 100
 ) -> Tuple[List[float], List[torch.Tensor]]:
-    bg = BatchGradient(plans)
+    bg = BatchGradient(plans, checkpoint_every)
     try:
         return bg(u0s, nsteps)
     finally:

@@ -3297,6 +3297,12 @@ static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s
   return JSD_OK;
 }
 
+// the three substep states the nonlinear adjoint recomputes, kept by the plan
+static int recompute_alloc(jsd_plan* p) {
+  if (p->SS) return JSD_OK;
+  return dalloc(p, (void**)&p->SS, 9 * p->fs * sizeof(double2));
+}
+
 extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, double dPdx, void* stream) {
   if (!p || !state || !grad) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   if (p->d.kind == JSD_KIND_PERTURBATION && p->d.linearise) return jsd_step_vjp(p, grad, 1, stream);
@@ -3305,10 +3311,8 @@ extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, do
   if (p->HGC) JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint has no coupling_term = False transpose");
   hipStream_t st = (hipStream_t)stream;
   const size_t sb = 3 * p->fs * sizeof(double2);
-  if (!p->SS) {
-    int rc = dalloc(p, (void**)&p->SS, 3 * sb);
-    if (rc) return rc;
-  }
+  int rc0 = recompute_alloc(p);
+  if (rc0) return rc0;
   // substep states 0, 1, 2 of the step
   double2* S0 = p->SS;
   double2* S1 = p->SS + 3 * p->fs;
@@ -3441,6 +3445,12 @@ struct jsd_batch {
   hipStream_t cs = nullptr;
   hipGraphExec_t xf = nullptr, xb = nullptr;
   hipGraph_t gf = nullptr, gb = nullptr;
+  // nonlinear runs: each run steps on its own stream, forked from and joined
+  // back to the caller's stream, so the runs' stage kernels overlap
+  bool concurrent = false;
+  std::vector<hipStream_t> rs;
+  std::vector<hipEvent_t> rev;
+  hipEvent_t fork = nullptr;
 };
 
 static int bdalloc(jsd_batch* b, void** ptr, size_t bytes) {
@@ -3542,6 +3552,9 @@ static void tune_stage(jsd_batch* b, int kind, const BatchStage* S, int nS, hipS
 
 extern "C" int jsd_batch_destroy(jsd_batch* b) {
   if (!b) return JSD_OK;
+  for (hipStream_t s : b->rs) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : b->rev) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->xf) (void)hipGraphExecDestroy(b->xf);
   if (b->xb) (void)hipGraphExecDestroy(b->xb);
   if (b->gf) (void)hipGraphDestroy(b->gf);
@@ -3575,6 +3588,65 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   jsd_batch* b = new jsd_batch();
   b->nrun = nrun;
   b->vjp = adjoints != nullptr;
+  // nonlinear perturbation runs step concurrently on their own streams
+  {
+    int nl = 0;
+    for (int r = 0; r < nrun; ++r)
+      if (plans[r] && plans[r]->d.kind == JSD_KIND_PERTURBATION && !plans[r]->d.linearise) ++nl;
+    if (nl > 0 && nl < nrun) {
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_STATE, "a batch holds linearised runs or nonlinear perturbation runs, not both");
+    }
+    b->concurrent = nl == nrun;
+  }
+  if (b->concurrent) {
+    for (int r = 0; r < nrun; ++r) {
+      jsd_plan* p = plans[r];
+      if (!states[r] || (b->vjp && !adjoints[r])) {
+        jsd_batch_destroy(b);
+        JSD_FAIL(JSD_ERR_ARG, "run %d: missing plan or buffer", r);
+      }
+      for (int q = 0; q < r; ++q)
+        if (plans[q] == p || states[q] == states[r] || (b->vjp && adjoints[q] == adjoints[r]) ||
+            (b->vjp && (adjoints[q] == states[r] || states[q] == adjoints[r]))) {
+          jsd_batch_destroy(b);
+          JSD_FAIL(JSD_ERR_ARG, "runs %d and %d share a plan or a buffer: each run needs its own", q, r);
+        }
+      if (b->vjp && (adjoints[r] == states[r] || !p->adjoint || p->HGC)) {
+        jsd_batch_destroy(b);
+        JSD_FAIL(JSD_ERR_STATE, "run %d: the adjoint needs its own buffer and a plan created with adjoint = 1 "
+                                "(coupling term on)", r);
+      }
+      if (b->vjp) {
+        const int rc = recompute_alloc(p);
+        if (rc) {
+          jsd_batch_destroy(b);
+          return rc;
+        }
+      }
+      b->plans.push_back(p);
+      b->state.push_back((double2*)states[r]);
+      b->adj.push_back(b->vjp ? (double2*)adjoints[r] : nullptr);
+      hipStream_t s = nullptr;
+      hipEvent_t e = nullptr;
+      if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        if (s) (void)hipStreamDestroy(s);
+        jsd_batch_destroy(b);
+        JSD_FAIL(JSD_ERR_HIP, "stream or event creation failed");
+      }
+      b->rs.push_back(s);
+      b->rev.push_back(e);
+    }
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+      b->fork = nullptr;
+      jsd_batch_destroy(b);
+      JSD_FAIL(JSD_ERR_HIP, "event creation failed");
+    }
+    for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = b->total[k] = 0;
+    *out = b;
+    return JSD_OK;
+  }
   for (int r = 0; r < nrun; ++r) {
     jsd_plan* p = plans[r];
     if (!p || !states[r] || (b->vjp && !adjoints[r])) {
@@ -3751,8 +3823,27 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   return JSD_OK;
 }
 
+// Runs fn(r, run stream) for every run on its own stream, after the work
+// already queued on st and before anything queued on st later.
+template <class F>
+static int batch_concurrent(jsd_batch* b, hipStream_t st, F&& fn) {
+  HIPCHK(hipEventRecord(b->fork, st));
+  for (int r = 0; r < b->nrun; ++r) {
+    HIPCHK(hipStreamWaitEvent(b->rs[r], b->fork, 0));
+    const int rc = fn(r, b->rs[r]);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(b->rev[r], b->rs[r]));
+  }
+  for (int r = 0; r < b->nrun; ++r) HIPCHK(hipStreamWaitEvent(st, b->rev[r], 0));
+  return JSD_OK;
+}
+
 extern "C" int jsd_batch_step(jsd_batch* b, int nsteps, void* stream) {
   if (!b || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (b->concurrent)
+    return batch_concurrent(b, (hipStream_t)stream, [&](int r, hipStream_t s) {
+      return jsd_step(b->plans[r], b->state[r], 0.0, nsteps, s);
+    });
   for (int n = 0; n < nsteps; ++n) HIPCHK(hipGraphLaunch(b->xf, (hipStream_t)stream));
   return JSD_OK;
 }
@@ -3760,8 +3851,21 @@ extern "C" int jsd_batch_step(jsd_batch* b, int nsteps, void* stream) {
 extern "C" int jsd_batch_step_vjp(jsd_batch* b, int nsteps, void* stream) {
   if (!b || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   if (!b->vjp) JSD_FAIL(JSD_ERR_STATE, "batch created without adjoint buffers");
+  if (b->concurrent) JSD_FAIL(JSD_ERR_STATE, "the adjoint of nonlinear runs takes their forward states: "
+                                             "jsd_batch_step_vjp_state");
   for (int n = 0; n < nsteps; ++n) HIPCHK(hipGraphLaunch(b->xb, (hipStream_t)stream));
   return JSD_OK;
+}
+
+extern "C" int jsd_batch_step_vjp_state(jsd_batch* b, const void* const* states, void* stream) {
+  if (!b || !states) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (!b->vjp) JSD_FAIL(JSD_ERR_STATE, "batch created without adjoint buffers");
+  if (!b->concurrent) return jsd_batch_step_vjp(b, 1, stream);  // the linearised transpose needs no state
+  for (int r = 0; r < b->nrun; ++r)
+    if (!states[r]) JSD_FAIL(JSD_ERR_ARG, "run %d: missing forward state", r);
+  return batch_concurrent(b, (hipStream_t)stream, [&](int r, hipStream_t s) {
+    return jsd_step_vjp_state(b->plans[r], states[r], b->adj[r], 0.0, s);
+  });
 }
 
 extern "C" int jsd_batch_info(jsd_batch* b, int* grid, int* blocks) {

@@ -1,5 +1,7 @@
-"""Batches of independent linearised runs (jsd_batch_*): one launch per stage
-for runs of uneven shape, Re, dt and box, replayed from a captured HIP graph.
+"""Batches of independent runs (jsd_batch_*).  Linearised runs: one launch per
+stage for runs of uneven shape, Re, dt and box, replayed from a captured HIP
+graph.  Nonlinear runs: concurrent streams, checked bit for bit against
+single runs and their gradient against the oracle.
 
 * every replay equals stepping each plan on its own, with eager work and
   synchronisations between the replays (the same kernel bodies; inlined into
@@ -130,9 +132,14 @@ def test_batch_rejects_unsupported_plans():
     u = torch.zeros(ns.shape, dtype=torch.complex128, device=DEV)
     with pytest.raises(_lib.JsdError):
         BatchPlan([ns], [u])
+    # linearised and nonlinear runs do not share a batch
     nonlin = StepPlan(g, 2000.0, 0.01, KIND_PERTURBATION, linearise=False)
+    lin = StepPlan(g, 2000.0, 0.01, KIND_PERTURBATION, linearise=True)
     with pytest.raises(_lib.JsdError):
-        BatchPlan([nonlin], [u])
+        BatchPlan([nonlin, lin], [u, torch.zeros_like(u)])
+    # the nonlinear adjoint needs plans built with adjoint = True
+    with pytest.raises(_lib.JsdError):
+        BatchPlan([nonlin], [u], [torch.zeros_like(u)])
 
 
 def test_library_rejects_repeated_plans_in_a_batch():
@@ -199,3 +206,90 @@ def test_cfg3_seeded_64_run_batch():
     print("run %d %s vs oracle after 2 steps: %.3e" % (k, shape, err))
     assert err < 2e-11
     batch.close()
+
+
+def _cfg4_reduced(n, adjoint=True):
+    """Reduced copies of the nonlinear gradient workload: 24 x 65 x 16
+    requested modes, Re 3000, Lx 2 pi, Lz pi, dt 0.005, initial energies
+    log-spaced in [1e-6, 1e-2] (seeded fields), one plan per run."""
+    from jax_spectral_dns_amd import initial
+
+    g = DM.make_grid((24, 65, 16), 2 * np.pi, np.pi)
+    plans, u0 = [], []
+    for r, e0 in enumerate(np.logspace(-6, -2, n)):
+        plans.append(StepPlan(g, 3000.0, 0.005, KIND_PERTURBATION, linearise=False, adjoint=adjoint))
+        u0.append(torch.from_numpy(initial.perturbation(g, seed=70 + r, energy=float(e0))).to(DEV))
+    return g, plans, u0
+
+
+def test_nonlinear_batch_steps_equal_individual_steps():
+    """A batch of nonlinear perturbation runs steps them concurrently, one
+    stream each; every run is bit-identical to stepping its plan alone."""
+    _, plans, u0 = _cfg4_reduced(8, adjoint=False)
+    ind = [u.clone() for u in u0]
+    for p, u in zip(plans, ind):
+        p.step(u, 0.0, 6)
+    bat = [u.clone() for u in u0]
+    batch = BatchPlan(plans, bat)
+    assert all(v["grid"] == 0 for v in batch.info().values())
+    batch.step(2)
+    batch.step(4)
+    torch.cuda.synchronize()
+    for a, b in zip(bat, ind):
+        assert torch.equal(a, b)
+    batch.close()
+
+
+def test_nonlinear_batch_gradient_matches_single_runs_and_oracle():
+    """The batched gain gradient of 8 nonlinear runs (checkpoint every 4 of 10
+    steps, recompute) equals each run's own discrete adjoint, and run 0 and
+    run 7 match autograd of the oracle (the stand-in for jax.grad) <= 1e-9."""
+    g, plans, u0 = _cfg4_reduced(8)
+    n = 10
+    J, G = batch_gain_and_grad(plans, u0, n, checkpoint_every=4)
+    for r in range(8):
+        j1, g1 = gain_and_grad(u0[r], plans[r], n, checkpoint_every=4)
+        assert J[r] == j1
+        assert torch.equal(G[r], g1)
+    d = O.make_domain((24, 65, 16), 2 * np.pi, np.pi)
+    sp = O.make_perturbation(d, 3000.0, 0.005, False)
+    for r in (0, 7):
+        x = torch.view_as_real(u0[r].cpu()).clone().requires_grad_(True)
+        u = torch.view_as_complex(x)
+        Jr = O.energy(d, O.run(sp, u, n)) / O.energy(d, u)
+        Jr.backward()
+        err = rel(G[r], torch.view_as_complex(x.grad))
+        print("run %d (E0 %.1e): J %.15f vs %.15f, grad rel %.3e" % (r, float(O.energy(d, u0[r].cpu())), J[r], float(Jr), err))
+        assert abs(J[r] - float(Jr)) <= 1e-11 * abs(float(Jr))
+        assert err < 1e-9
+
+
+def test_nonlinear_gradient_full_size_against_finite_differences():
+    """One run of the nonlinear gradient workload at full size (96 x 129 x 80,
+    Re 3000, dt 0.005, E0 1e-2), 20 steps with a checkpoint every 10: the
+    directional derivative along two seeded directions against centred
+    differences (h = 1e-4 relative; the O(h^2) truncation is ~1e-8)."""
+    from jax_spectral_dns_amd import initial
+
+    g = DM.make_grid((96, 129, 80), 2 * np.pi, np.pi)
+    plan = StepPlan(g, 3000.0, 0.005, KIND_PERTURBATION, linearise=False, adjoint=True)
+    plan.ops = None
+    u0 = torch.from_numpy(initial.perturbation(g, seed=3, energy=1e-2)).to(DEV)
+    n = 20
+    J, grad = batch_gain_and_grad([plan], [u0], n, checkpoint_every=10)
+    J, grad = J[0], grad[0]
+
+    def gain(u):
+        v = u.clone()
+        plan.step(v, 0.0, n)
+        return plan.energy(v) / plan.energy(u)
+
+    assert abs(gain(u0) - J) <= 1e-13 * J
+    for k in range(2):
+        rng = np.random.default_rng(80 + k)
+        dirn = torch.from_numpy(initial.perturbation(g, seed=90 + k, energy=1e-2)).to(DEV)
+        h = 1e-4
+        fd = (gain(u0 + h * dirn) - gain(u0 - h * dirn)) / (2 * h)
+        ad = float(torch.sum(grad.real * dirn.real + grad.imag * dirn.imag))
+        print("dir %d fd %.10e ad %.10e rel %.2e" % (k, fd, ad, abs(fd - ad) / abs(ad)))
+        assert abs(fd - ad) < 1e-6 * abs(ad)
