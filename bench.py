@@ -249,25 +249,32 @@ def batch_bench(nsteps: int) -> dict:
     }
 
 
-def adjoint_bench(nsteps: int) -> dict:
-    """One gain-gradient evaluation of the nonlinear perturbation step at
-    96 x 129 x 80 modes (Re 3000, Lx 2 pi, Lz pi, dt 0.005, E0 1e-2): forward
-    steps with a checkpoint every 10, then the discrete adjoint with
-    recompute; per-step costs and the wall time of the whole evaluation."""
+def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
+    """Gain gradients of the nonlinear perturbation step at 96 x 129 x 80
+    modes (Re 3000, Lx 2 pi, Lz pi, dt 0.005): the batch of nrun runs with
+    initial energies log-spaced in [1e-6, 1e-2] (seeded fields), each
+    gradient over nsteps steps with a checkpoint every 10 and recompute;
+    one run at a time (vjp.gain_and_grad) against the concurrent batch
+    (batch.BatchGradient), plus the per-step costs of one run."""
     import torch
 
     from jax_spectral_dns_amd import KIND_PERTURBATION
     from jax_spectral_dns_amd import domain_math as DM
     from jax_spectral_dns_amd import initial
+    from jax_spectral_dns_amd.batch import BatchGradient
     from jax_spectral_dns_amd.plan import StepPlan
     from jax_spectral_dns_amd.vjp import gain_and_grad
 
     g = DM.make_grid((96, 129, 80), 2 * np.pi, np.pi)
     t0 = time.time()
-    plan = StepPlan(g, 3000.0, 0.005, KIND_PERTURBATION, linearise=False, adjoint=True)
-    plan.ops = None
+    ops = DM.build_operators(g, 3000.0, 0.005, full=False)
+    plans = [StepPlan(g, 3000.0, 0.005, KIND_PERTURBATION, linearise=False, adjoint=True, ops=ops) for _ in range(nrun)]
+    for p in plans:
+        p.ops = None
     t_build = time.time() - t0
-    u0 = torch.from_numpy(initial.perturbation(g, seed=0, energy=1e-2)).cuda()
+    u0s = [torch.from_numpy(initial.perturbation(g, seed=r, energy=float(e))).cuda()
+           for r, e in enumerate(np.logspace(-6, -2, nrun))]
+    plan, u0 = plans[-1], u0s[-1]
     u = u0.clone()
     plan.step(u, 0.0, 2)  # warm-up
     torch.cuda.synchronize()
@@ -281,16 +288,37 @@ def adjoint_bench(nsteps: int) -> dict:
     J, grad = gain_and_grad(u0, plan, nsteps, checkpoint_every=10)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    # the batch: one run at a time, then concurrently
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = [gain_and_grad(v, p, nsteps, checkpoint_every=10) for p, v in zip(plans, u0s)]
+    torch.cuda.synchronize()
+    t_one = time.perf_counter() - t0
+    bg = BatchGradient(plans, checkpoint_every=10)
+    bg(u0s, 2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Jb, Gb = bg(u0s, nsteps)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    bg.close()
+    gdiff = max(float(torch.linalg.norm(a - b[1]) / torch.linalg.norm(b[1])) for a, b in zip(Gb, ref))
     finite = bool(torch.isfinite(grad).all())
-    plan.close()
+    for p in plans:
+        p.close()
     return {
         "workload": "NavierStokesVelVortPerturbation nonlinear, 96x129x80 modes, Re 3000, Lx 2pi, Lz pi, dt 0.005, "
-        "E0 1e-2, gain gradient over %d steps, checkpoint every 10 with recompute" % nsteps,
+        "gain gradient over %d steps, checkpoint every 10 with recompute; batch of %d runs with E0 log-spaced "
+        "in [1e-6, 1e-2]" % (nsteps, nrun),
         "forward_ms_per_step": 1e3 * t_fwd / nsteps,
         "gradient_eval_s": t_grad,
         "adjoint_ms_per_step_incl_recompute": 1e3 * (t_grad - t_fwd) / nsteps,
         "gain": J,
         "grad_finite": finite,
+        "grad_evals_per_s_batched": nrun / t_bat,
+        "grad_evals_per_s_one_at_a_time": nrun / t_one,
+        "grad_speedup": t_one / t_bat,
+        "max_rel_diff_grad_batched_vs_single": gdiff,
         "plan_build_s": round(t_build, 2),
     }
 

@@ -99,6 +99,119 @@ class SteepestAdaptiveDescentSolver:
         return self.values
 
 
+class MultiStartSteepestAdaptiveDescentSolver:
+    """SteepestAdaptiveDescentSolver (gradient_descent_solver.py:340-487) on
+    several starts at once, with every objective and gradient evaluation of
+    all starts batched: one batched forward of the linearised perturbation
+    step (jsd_batch_step over the starts' plans) gives every candidate's
+    gain, and one batched dual (the linearised dual plans of the same
+    starts) gives every v(0) for the projected gradients
+    (get_projected_grad, navier_stokes_perturbation_dual.py:1122-1167).
+
+    Each start keeps its own guess, step size and sub-iteration count and
+    follows the single-start rules: a candidate is accepted when its gain
+    rose (the step size grows by 1.5 and the gradient is taken at the new
+    guess), else the step size shrinks by 5 and the direction is re-projected
+    from the last accepted guess, at most max_number_of_sub_iterations times
+    per iteration.  The starts advance in lockstep rounds; a start whose
+    iteration is finished waits (its candidate is its current guess).  The
+    starts may differ in grid, Re, box and initial field; they share the
+    number of time steps.  Linearised problems only (the batched dual is the
+    linearised one)."""
+
+    def __init__(self, nses, **params: Any):
+        from .batch import BatchPlan
+        from .navier_stokes_perturbation_dual import projected_gradient
+
+        self._projected = projected_gradient
+        self.nses = list(nses)
+        if not self.nses:
+            raise ValueError("no starts")
+        if not all(bool(n.linearise(0)) for n in self.nses):
+            raise ValueError("the batched dual is the linearised one: every start must be linearised")
+        counts = {n.number_of_time_steps() for n in self.nses}
+        if len(counts) != 1:
+            raise ValueError("the starts must share the number of time steps")
+        self.nsteps = counts.pop()
+        self.step_size = [float(params.get("step_size", 1e-2))] * len(self.nses)
+        self.number_of_steps = int(params.get("max_iter", 20))
+        self.max_number_of_sub_iterations = int(params.get("max_number_of_sub_iterations", 10))
+        self.fplans = [n.plan() for n in self.nses]
+        self.dplans = [NavierStokesVelVortPerturbationDual(n).plan_for_step(0) for n in self.nses]
+        dev = torch.device("cuda")
+        self.u = [torch.zeros(p.shape, dtype=torch.complex128, device=dev) for p in self.fplans]
+        self.v = [torch.zeros_like(x) for x in self.u]
+        self.fwd = BatchPlan(self.fplans, self.u)
+        self.dual = BatchPlan(self.dplans, self.v)
+        self.current_guess = [n._state(n.get_initial_field("velocity_hat").get_data()).clone() for n in self.nses]
+        self.e_0 = [p.energy(g) for p, g in zip(self.fplans, self.current_guess)]
+        self.value = self._gains(self.current_guess)
+        self.u0 = [g.clone() for g in self.current_guess]
+        self.v0 = self._duals()
+        self.grad = [self._projected(p, a, b, s) for p, a, b, s in zip(self.fplans, self.u0, self.v0, self.step_size)]
+        self.values: List[List[float]] = [list(self.value)]
+        self.i = 0
+        self.evaluations = 1
+
+    def _gains(self, guesses: List[torch.Tensor]) -> List[float]:
+        """Batched forward: J_r = E(u_r(T)) / E(u_r(0)); leaves u_r(T) in self.u."""
+        for u, g in zip(self.u, guesses):
+            u.copy_(g)
+        self.fwd.step(self.nsteps)
+        return [p.energy(u) / p.energy(g) for p, u, g in zip(self.fplans, self.u, guesses)]
+
+    def _duals(self) -> List[torch.Tensor]:
+        """Batched dual from v(T) = -u(T) of the last forward: v_r(0)."""
+        for v, u in zip(self.v, self.u):
+            torch.neg(u, out=v)
+        self.dual.step(self.nsteps)
+        return [v.clone() for v in self.v]
+
+    def normalize_field(self, r: int, v: torch.Tensor) -> torch.Tensor:
+        return v * (self.e_0[r] / self.fplans[r].energy(v)) ** 0.5
+
+    def update(self) -> None:
+        R = len(self.nses)
+        searching = [True] * R
+        sub = [0] * R
+        while any(searching):
+            cand = [
+                self.normalize_field(r, self.current_guess[r] + self.step_size[r] * self.grad[r])
+                if searching[r] else self.current_guess[r]
+                for r in range(R)
+            ]
+            gains = self._gains(cand)
+            v0s = self._duals()
+            self.evaluations += 1
+            for r in range(R):
+                if not searching[r]:
+                    continue
+                if gains[r] - self.value[r] > 0.0:
+                    self.step_size[r] *= 1.5
+                    self.current_guess[r] = cand[r]
+                    self.value[r] = gains[r]
+                    self.u0[r], self.v0[r] = cand[r], v0s[r]
+                    self.grad[r] = self._projected(self.fplans[r], self.u0[r], self.v0[r], self.step_size[r])
+                    searching[r] = False
+                    continue
+                self.step_size[r] /= 5.0
+                self.grad[r] = self._projected(self.fplans[r], self.u0[r], self.v0[r], self.step_size[r])
+                sub[r] += 1
+                if sub[r] > self.max_number_of_sub_iterations:
+                    searching[r] = False
+        self.values.append(list(self.value))
+        self.i += 1
+
+    def optimise(self, iterations: Optional[int] = None) -> List[List[float]]:
+        for _ in range(self.number_of_steps if iterations is None else iterations):
+            self.update()
+        return self.values
+
+    def close(self) -> None:
+        self.fwd.close()
+        self.dual.close()
+
+
 class ConjugateGradientDescentSolver:
     """Restates ConjugateGradientDescentSolver (gradient_descent_solver.py:490-860)
     on its default path (use_linesearch = False).

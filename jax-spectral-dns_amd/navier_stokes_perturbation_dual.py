@@ -301,17 +301,8 @@ class NavierStokesVelVortPerturbationDual:
         """Energy-constrained update direction (navier_stokes_perturbation_dual.py:1122-1167)."""
         if self.v_0 is None:
             self.run_backward_calculation()
-        f = 0.9 if step_size_factor is None else step_size_factor
-        p = self.forward_equation.plan()
-        u0, v0 = self.velocity_u_hat_0, self.v_0
-        e0 = p.energy(u0)
-        e0_adj = p.energy(v0)
-        c0 = 2 * 0.5 * (p.energy(u0 + v0) - e0 - e0_adj)  # 2 <u0, v0>_E
-        max_step = f * (e0_adj / e0 - c0**2 / (4 * e0**2)) ** (-1 / 2)
-        step_size = min(step_size, max_step)
-        A = (step_size * c0 - 2 * e0) / (2 * step_size * e0)
-        lam = A + math.sqrt(A**2 - (step_size * e0_adj - c0) / (step_size * e0))
-        return lam * u0 - v0, True
+        return projected_gradient(self.forward_equation.plan(), self.velocity_u_hat_0, self.v_0, step_size,
+                                  step_size_factor), True
 
 
     def get_projected_cg_grad(
@@ -351,6 +342,23 @@ class NavierStokesVelVortPerturbationDual:
         e0_adj = p.energy(v0)
         c0 = 2 * 0.5 * (p.energy(u0 + v0) - e0 - e0_adj)
         return (e0_adj / e0 - c0**2 / (4 * e0**2)) ** (-1 / 2)
+
+
+def projected_gradient(
+    plan: StepPlan, u0: torch.Tensor, v0: torch.Tensor, step_size: float, step_size_factor: Optional[float] = None
+) -> torch.Tensor:
+    """lam u0 - v0 with lam such that a step of step_size (capped at
+    step_size_factor times the largest admissible one) keeps the energy of
+    u0 (get_projected_grad_from_u_and_v, navier_stokes_perturbation_dual.py:1122-1167)."""
+    f = 0.9 if step_size_factor is None else step_size_factor
+    e0 = plan.energy(u0)
+    e0_adj = plan.energy(v0)
+    c0 = 2 * 0.5 * (plan.energy(u0 + v0) - e0 - e0_adj)  # 2 <u0, v0>_E
+    max_step = f * (e0_adj / e0 - c0**2 / (4 * e0**2)) ** (-1 / 2)
+    step_size = min(step_size, max_step)
+    A = (step_size * c0 - 2 * e0) / (2 * step_size * e0)
+    lam = A + math.sqrt(A**2 - (step_size * e0_adj - c0) / (step_size * e0))
+    return lam * u0 - v0
 
 
 def perform_step_navier_stokes_perturbation_dual(nse: NavierStokesVelVortPerturbation, step_size: float = 1e-2):
