@@ -1570,6 +1570,7 @@ __device__ void mv_mfma(const double* __restrict__ AT, int N, const double* X, d
 }
 
 #include "jsd_solve.inc"
+#include "jsd_solve8.inc"
 
 #include "jsd_adjoint.inc"
 #include "jsd_adjoint_nl.inc"
@@ -1673,6 +1674,7 @@ struct jsd_plan {
   double2 *twx, *twz;
   FftPlan fpx, fpz;
   int generic_fft;  // 1: run-time-length FFT kernels only (JSD_FFT_GENERIC=1, a diagnostic switch)
+  int solve8;       // eigen form: 8 pencils per solve workgroup (0 with JSD_SOLVE4=1, a diagnostic switch)
   // workspaces
   double2 *DU, *H, *HV, *HG, *HVo, *HGo, *CONV, *CONVo, *T, *V, *WF, *DWF;
   double* partial;
@@ -2195,6 +2197,8 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   {
     const char* env = getenv("JSD_FFT_GENERIC");
     p->generic_fft = (env && env[0] == '1') ? 1 : 0;
+    const char* env4 = getenv("JSD_SOLVE4");
+    p->solve8 = (env4 && env4[0] == '1') ? 0 : 1;
   }
   p->M = M;
   p->N = N;
@@ -2815,7 +2819,9 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
     hipLaunchKernelGGL(k_step_terms, dim3((unsigned)((p->fs + 255) / 256)), dim3(256), 0, st, T);
     HIPCHK(hipGetLastError());
   }
-  const int nblk = ((solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
+  const bool s8 = p->EVF && p->solve8;
+  const int groups = s8 ? solve_groups8(p->M, p->Z) : solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb);
+  const int nblk = ((groups + 7) / 8) * 8;
   SolveArgs S = solve_args(p, U, s, dPdx, roles_of(p));
   if (x) {
     S.HGC = p->HGS;
@@ -2828,6 +2834,12 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
   prof_begin(p, 5, st);
   if (!p->EVF)
     hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, S);
+  else if (s8 && !S.DUo)
+    hipLaunchKernelGGL(k_solve_eig8<0>, dim3(nblk), dim3(256), 0, st, S);
+  else if (s8 && S.du_full)
+    hipLaunchKernelGGL(k_solve_eig8<2>, dim3(nblk), dim3(256), 0, st, S);
+  else if (s8)
+    hipLaunchKernelGGL(k_solve_eig8<1>, dim3(nblk), dim3(256), 0, st, S);
   else if (!S.DUo)
     hipLaunchKernelGGL(k_solve_eig<0>, dim3(nblk), dim3(256), 0, st, S);
   else if (S.du_full)
