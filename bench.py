@@ -39,7 +39,7 @@ KERNEL_NAMES = {
     "fft_z_mid": "k_fft_z_mid_256<6, 0, true, 2>",
     "fft_x_fwd": "k_fft_x_fwd_t<Rad<8, 5, 5, 1> >",
     "ygemm_hv": "k_ygemm<1>",
-    "solve": "k_solve_eig",
+    "solve": "k_solve_eig8",
     "nl_lin": "k_nl_lin",
 }
 
@@ -337,8 +337,13 @@ def main() -> int:
     ap.add_argument(
         "--copy-probe",
         action="store_true",
-        help="run one 256 MiB device-to-device elementwise copy (torch.neg) first: the HBM-counter "
-        "calibration kernel for profiler runs",
+        help="run one 1 GiB device-to-device elementwise copy (torch.neg, four times the last-level cache) "
+        "first: the HBM-counter calibration kernel for profiler runs",
+    )
+    ap.add_argument(
+        "--only",
+        choices=("adjoint", "batch"),
+        help="run only the nonlinear-gradient or the batched-runs figure (for kernel traces of those paths)",
     )
     args = ap.parse_args()
     if args.gpus != 1:
@@ -359,6 +364,12 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
+    if args.only == "adjoint":
+        print(json.dumps({"nonlinear_gradient": adjoint_bench(args.adjoint_steps)}), flush=True)
+        return 0
+    if args.only == "batch":
+        print(json.dumps({"batch": batch_bench(args.batch_steps)}), flush=True)
+        return 0
     shape, lx, lz, re, dt = (128, 129, 128), 4 * np.pi, 2 * np.pi, 5000.0, 0.005
     dpdx = -2.0 / re
     g = DM.make_grid(shape, lx, lz)
@@ -373,7 +384,7 @@ def main() -> int:
     u = u0_t.to(dev)
     lib = _lib.load()
     if args.copy_probe:
-        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
         torch.neg(src, out=dst)
         torch.cuda.synchronize()
