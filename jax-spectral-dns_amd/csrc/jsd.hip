@@ -1183,6 +1183,8 @@ typedef Rad<8, 8> RZ64;
 typedef Rad<8, 8, 2> RZ128;
 typedef Rad<8, 8, 4> RZ256;
 
+#include "jsd_fft_x.inc"
+
 template <class C>
 static bool plan_is(const FftPlan& f) {
   if (f.p != C::P || f.nrad != C::NS) return false;
@@ -1675,6 +1677,8 @@ struct jsd_plan {
   FftPlan fpx, fpz;
   int generic_fft;  // 1: run-time-length FFT kernels only (JSD_FFT_GENERIC=1, a diagnostic switch)
   int solve8;       // eigen form: 8 pencils per solve workgroup (0 with JSD_SOLVE4=1, a diagnostic switch)
+  int xreg;         // x-pass kernels: 1 (default) the k_fft_x_*_r kernels, inverse inputs scattered through LDS;
+                    // 2 the inverse's first stage in registers too; 0 the k_fft_x_*_t kernels (JSD_XPASS=0/1/2)
   // workspaces
   double2 *DU, *H, *HV, *HG, *HVo, *HGo, *CONV, *CONVo, *T, *V, *WF, *DWF;
   double* partial;
@@ -2199,6 +2203,8 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     p->generic_fft = (env && env[0] == '1') ? 1 : 0;
     const char* env4 = getenv("JSD_SOLVE4");
     p->solve8 = (env4 && env4[0] == '1') ? 0 : 1;
+    const char* envx = getenv("JSD_XPASS");
+    p->xreg = (envx && envx[0] >= '0' && envx[0] <= '2') ? envx[0] - '0' : 1;
   }
   p->M = M;
   p->N = N;
@@ -2564,6 +2570,33 @@ static NlLinArgs nl_lin_args(const jsd_plan* p, const double2* U) {
 }
 
 // H (3 comps, stored convention) from the state U (and the forward state for a nonlinear dual)
+// The register x kernels take even transforms as (A, d_x A) and odd ones as
+// (A, d_z B); every set built here has that form.
+static bool xset_alternates(const XInvSet& set, int ntr) {
+  for (int t = 0; t < ntr; ++t) {
+    if (set.opA[t] != OP_ID) return false;
+    if ((t & 1) == 0 && (set.opB[t] != OP_DX || set.A[t] != set.B[t])) return false;
+    if ((t & 1) == 1 && set.opB[t] != OP_DZ) return false;
+  }
+  return true;
+}
+
+static void launch_x_inv_ct(jsd_plan* p, const XInvSet& set, int ntr, int nqb, dim3 grid, size_t lds, hipStream_t st,
+                            bool* done) {
+  const bool reg = p->xreg && xset_alternates(set, ntr);
+  *done = visit_x(p->fpx, [&](auto c) {
+    if (reg && p->xreg == 2)
+      hipLaunchKernelGGL((k_fft_x_inv_r<decltype(c), true>), grid, dim3(256), lds, st, set, ntr, nqb, p->fpx.tw, p->kx,
+                         p->kz, p->M, p->N, p->Z, p->ZT, p->s_inv);
+    else if (reg)
+      hipLaunchKernelGGL((k_fft_x_inv_r<decltype(c), false>), grid, dim3(256), lds, st, set, ntr, nqb, p->fpx.tw,
+                         p->kx, p->kz, p->M, p->N, p->Z, p->ZT, p->s_inv);
+    else
+      hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, grid, dim3(256), lds, st, set, ntr, nqb, p->fpx.tw, p->kx, p->kz,
+                         p->M, p->N, p->Z, p->ZT, p->s_inv);
+  });
+}
+
 static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStream_t st, bool have_du) {
   const jsd_plan_desc& d = p->d;
   const size_t fs = p->fs;
@@ -2627,10 +2660,8 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   const bool ct = !p->generic_fft;
   prof_begin(p, 1, st);
   const dim3 grid_xi(8 * ((ntr * nqb * N + 7) / 8));
-  const bool xi_ct = ct && visit_x(p->fpx, [&](auto c) {
-    hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx.tw, p->kx,
-                       p->kz, M, N, Z, ZT, p->s_inv);
-  });
+  bool xi_ct = false;
+  if (ct) launch_x_inv_ct(p, set, ntr, nqb, grid_xi, lds_x, st, &xi_ct);
   if (!xi_ct)
     hipLaunchKernelGGL(k_fft_x_inv, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx, p->kx, p->kz, M, N, Z, ZT,
                        p->s_inv);
@@ -2684,8 +2715,12 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   prof_begin(p, 3, st);
   const dim3 grid_xf(8 * ((2 * nqb * N + 7) / 8));
   const bool xf_ct = ct && visit_x(p->fpx, [&](auto c) {
-    hipLaunchKernelGGL(k_fft_x_fwd_t<decltype(c)>, grid_xf, dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs,
-                       p->fpx.tw, M, N, Z, ZT, nqb, p->s_fwd);
+    if (p->xreg)
+      hipLaunchKernelGGL(k_fft_x_fwd_r<decltype(c)>, grid_xf, dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs,
+                         p->fpx.tw, M, N, Z, ZT, nqb, p->s_fwd);
+    else
+      hipLaunchKernelGGL(k_fft_x_fwd_t<decltype(c)>, grid_xf, dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs,
+                         p->fpx.tw, M, N, Z, ZT, nqb, p->s_fwd);
   });
   if (!xf_ct)
     hipLaunchKernelGGL(k_fft_x_fwd, grid_xf, dim3(256), lds_x, st, p->V, (long)ts, p->H, (long)fs, p->fpx, M, N, Z,
@@ -3227,10 +3262,8 @@ static int forward_inverse_x(jsd_plan* p, const double2* U, hipStream_t st) {
   for (int t = 0; t < 6; ++t) set.T[t] = p->T + t * ts;
   const int ntr = 6;
   const dim3 grid_xi(8 * ((ntr * nqb * N + 7) / 8));
-  const bool ok = visit_x(p->fpx, [&](auto c) {
-    hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx.tw, p->kx,
-                       p->kz, M, N, Z, ZT, p->s_inv);
-  });
+  bool ok = false;
+  launch_x_inv_ct(p, set, ntr, nqb, grid_xi, lds_x, st, &ok);
   if (!ok)
     hipLaunchKernelGGL(k_fft_x_inv, grid_xi, dim3(256), lds_x, st, set, ntr, nqb, p->fpx, p->kx, p->kz, M, N, Z, ZT,
                        p->s_inv);

@@ -43,11 +43,46 @@ __device__ __forceinline__ int dit_iperm(int n) {
   }
 }
 
+// Radix 3 and 5 through the symmetric sums and differences of the inputs
+// (v_m + v_{R-m}, v_m - v_{R-m}): out_q and out_{R-q} share the cosine part
+// and differ in the sign of the sine part, 12 real operations for radix 3
+// and 36 for radix 5 instead of the 16 and 80 of dft_generic.  The constants
+// are the correctly rounded cos / sin of 2 pi m / R of DftConst.
+template <int SIGN>
+__device__ __forceinline__ void dft3_sym(double2* v) {
+  constexpr double c1 = DftConst<3>::c[1], s1 = (double)SIGN * DftConst<3>::s[1];
+  const double2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+  const double2 a = make_double2(v[0].x + c1 * s.x, v[0].y + c1 * s.y);
+  const double2 b = make_double2(-s1 * d.y, s1 * d.x);  // i sign sin(2 pi / 3) d
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(a, b);
+  v[2] = csub(a, b);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft5_sym(double2* v) {
+  constexpr double c1 = DftConst<5>::c[1], c2 = DftConst<5>::c[2];
+  constexpr double s1 = (double)SIGN * DftConst<5>::s[1], s2 = (double)SIGN * DftConst<5>::s[2];
+  const double2 p1 = cadd(v[1], v[4]), m1 = csub(v[1], v[4]);
+  const double2 p2 = cadd(v[2], v[3]), m2 = csub(v[2], v[3]);
+  const double2 a1 = make_double2(v[0].x + c1 * p1.x + c2 * p2.x, v[0].y + c1 * p1.y + c2 * p2.y);
+  const double2 a2 = make_double2(v[0].x + c2 * p1.x + c1 * p2.x, v[0].y + c2 * p1.y + c1 * p2.y);
+  // i (s1 m1 + s2 m2) and i (s2 m1 - s1 m2)
+  const double2 b1 = make_double2(-(s1 * m1.y + s2 * m2.y), s1 * m1.x + s2 * m2.x);
+  const double2 b2 = make_double2(-(s2 * m1.y - s1 * m2.y), s2 * m1.x - s1 * m2.x);
+  v[0] = cadd(cadd(v[0], p1), p2);
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft_ct(double2* v) {
   constexpr double sign = (double)SIGN;
   if constexpr (R == 2) dft2(v);
+  else if constexpr (R == 3) dft3_sym<SIGN>(v);
   else if constexpr (R == 4) dft4(v, sign);
+  else if constexpr (R == 5) dft5_sym<SIGN>(v);
   else if constexpr (R == 8) dft8(v, sign);
   else dft_generic<R>(v, sign);
 }

@@ -287,8 +287,8 @@ def test_multistep_call_chains_derivatives(kind):
                                             (KIND_DUAL, True)])
 def test_eight_pencil_solve_matches_four_pencil_solve(monkeypatch, kind, linearise, shape):
     """The solve with eight pencils per workgroup (the default) performs the
-    four-pencil kernel's arithmetic in the same order (the compiler contracts
-    a few multiply-adds differently): three steps agree to rounding
+    four-pencil kernel's arithmetic in the same order, with reciprocals in
+    place of two divisions: three steps agree to rounding
     (<= 1e-14 per step, measured <= 3e-15), for odd and even Ny, a last group
     that is partly empty (16 x 4 gives 17 x 5 = 85 pencils), every
     derivative-output mode and the full size."""
@@ -321,3 +321,46 @@ def test_eight_pencil_solve_matches_four_pencil_solve(monkeypatch, kind, lineari
         err = rel(a, b)
         print("step", n, "8 vs 4 pencils rel", err)
         assert err <= 1e-14 * (n + 1), (n, err)
+
+
+# x transform lengths of the compile-time kernels: 40 (8 5), 75 (5 5 3),
+# 100 (5 5 4), 150 (5 5 3 2), 160 (8 5 4), 200 (8 5 5)
+X_SHAPES = [(24, 33, 16), (48, 33, 8), (64, 17, 8), (96, 17, 8), (104, 17, 8), (128, 33, 8)]
+
+
+@pytest.mark.parametrize("shape", X_SHAPES)
+@pytest.mark.parametrize("kind", [KIND_NS, KIND_PERTURBATION, KIND_DUAL])
+def test_register_x_passes_match_lds_x_passes(monkeypatch, kind, shape):
+    """The x passes of jsd_fft_x.inc (the default: the inverse's last stage
+    stored straight to T, the forward's first stage in registers, twiddle
+    powers; and, with JSD_XPASS=2, the inverse's first stage in registers
+    too) agree with the LDS x passes (JSD_XPASS=0) to rounding on every
+    compile-time x length, for the NS, perturbation and dual products; and
+    match the oracle's nonlinear term."""
+    g, d = pair(shape, 2 * np.pi, np.pi)
+    u = perturbation_field(g, seed=51, energy=1e-2)
+    uf = perturbation_field(g, seed=52, energy=1e-2)
+
+    def make():
+        if kind == KIND_NS:
+            return StepPlan(g, 5000.0, 0.005, KIND_NS)
+        if kind == KIND_PERTURBATION:
+            return StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=False)
+        fwd_ops = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION).ops
+        return StepPlan(g, 3000.0, 0.01, KIND_DUAL, linearise=False, ops=fwd_ops, explicit_dt=-0.01, plan_re=-3000.0)
+
+    out = {}
+    for mode in ("0", "1", "2"):
+        monkeypatch.setenv("JSD_XPASS", mode)
+        out[mode] = make().nonlinear(u.to(DEV), None if kind != KIND_DUAL else uf.to(DEV))
+    monkeypatch.delenv("JSD_XPASS", raising=False)
+    a = out["1"]
+    for mode in ("1", "2"):
+        for x, y in zip(out[mode], out["0"]):
+            err = rel(x, y)
+            print(shape, kind, "x-pass mode", mode, "vs LDS kernels rel", err)
+            assert err < 1e-13
+    if kind == KIND_NS:
+        ref = O.nl_ns_skew(d, u)
+        for x, r in zip((a[0], a[1], a[2]), (ref[0], ref[1], ref[3])):
+            assert rel(x, r) < 1e-12
