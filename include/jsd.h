@@ -140,6 +140,14 @@ int jsd_step_vjp_state(jsd_plan* plan, const void* state, void* grad, double dPd
  * navier_stokes_perturbation_instationary.py:39-80) is set before each step.
  * Not for plans in a batch while it steps. */
 int jsd_set_base_flow(jsd_plan* plan, const double* base_u, const double* base_du, void* stream);
+/* As jsd_set_base_flow with device arrays: two device-to-device copies
+ * queued on the stream, no synchronisation, so a loop of
+ * (jsd_set_base_flow_device, jsd_step) over a device table of profiles
+ * (one row per time step) never waits for the GPU and can be captured in a
+ * graph.  The arrays must stay valid until the stream has run the copies.
+ * Replaces the same reference interface (vel_base_fn evaluated per step,
+ * navier_stokes_perturbation_instationary.py:39-80). */
+int jsd_set_base_flow_device(jsd_plan* plan, const double* base_u, const double* base_du, void* stream);
 
 /* One RK3/CN step like jsd_step (forward plans, fwd_hat NULL) or
  * jsd_dual_step (dual plans) with the optional source term of
@@ -259,6 +267,13 @@ int jsd_batch_step_vjp_state(jsd_batch* batch, const void* const* states, void* 
  * transpose, D^T accumulation) and their total logical block counts (all 0
  * for a batch of nonlinear runs, which launches each run's own kernels). */
 int jsd_batch_info(jsd_batch* batch, int* grid, int* blocks);
+/* Tile shapes the autotuner chose, per stage as in jsd_batch_info: for the
+ * forward solve the pencils per workgroup (4: one 16-column MFMA tile per
+ * wave; 8: two tiles per wave, eigen-form plans only), timed together with
+ * the grid size on the batch's own data; 0 for stages with one shape and
+ * for batches of nonlinear runs.  Replaces nothing in the reference (which
+ * has no batching); it reports how the batch runs. */
+int jsd_batch_tiles(jsd_batch* batch, int* tile);
 
 const char* jsd_last_error(void);
 int jsd_version(void);

@@ -73,6 +73,7 @@ EXPORTS = (
     "jsd_energy_grad",
     "jsd_profile",
     "jsd_set_base_flow",
+    "jsd_set_base_flow_device",
     "jsd_step_source",
     "jsd_substep_terms",
     "jsd_laplacian",
@@ -85,6 +86,7 @@ EXPORTS = (
     "jsd_batch_step_vjp",
     "jsd_batch_step_vjp_state",
     "jsd_batch_info",
+    "jsd_batch_tiles",
     "jsd_last_error",
     "jsd_version",
 )

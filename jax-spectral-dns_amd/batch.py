@@ -109,8 +109,10 @@ class BatchPlan:
     def info(self) -> dict:
         g = (ctypes.c_int * len(STAGES))()
         b = (ctypes.c_int * len(STAGES))()
+        t = (ctypes.c_int * len(STAGES))()
         _lib.check(_lib.load().jsd_batch_info(self._h, g, b))
-        return {s: {"grid": g[i], "blocks": b[i]} for i, s in enumerate(STAGES)}
+        _lib.check(_lib.load().jsd_batch_tiles(self._h, t))
+        return {s: {"grid": g[i], "blocks": b[i], "tile": t[i]} for i, s in enumerate(STAGES)}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -3150,6 +3150,16 @@ extern "C" int jsd_set_base_flow(jsd_plan* p, const double* base_u, const double
   return JSD_OK;
 }
 
+extern "C" int jsd_set_base_flow_device(jsd_plan* p, const double* base_u, const double* base_du, void* stream) {
+  if (!p || !base_u || !base_du) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  if (p->d.kind == JSD_KIND_NS || !p->Ub) JSD_FAIL(JSD_ERR_STATE, "the plan has no base flow");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t b = (size_t)p->N * sizeof(double);
+  HIPCHK(hipMemcpyAsync(p->Ub, base_u, b, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(p->dUb, base_du, b, hipMemcpyDeviceToDevice, st));
+  return JSD_OK;
+}
+
 // ---------------------------------------------------------------------------
 // discrete adjoint of the linearised perturbation step
 // ---------------------------------------------------------------------------
@@ -3444,6 +3454,17 @@ struct OpSolve {
       solve_body_eig<1>(a, k.x);  // batches hold linearised runs: D v only
   }
 };
+// the same solve with eight pencils per logical block (two column tiles per
+// wave); eigen form only
+struct OpSolve8 {
+  typedef SolveArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) {
+    if (!a.DUo)
+      solve_body_eig8<0>(a, k.x);
+    else
+      solve_body_eig8<1>(a, k.x);
+  }
+};
 struct OpSolveT {
   typedef SolveTArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { solve_T_body(a, k.x); }
@@ -3484,6 +3505,8 @@ struct jsd_batch {
   std::vector<double2*> state, adj;
   std::vector<void*> allocs;
   BatchStage yg, nl, solve[3];          // forward: y-GEMM (D v), linearised term, solve per substep
+  BatchStage solve8[3];                 // the solve with eight pencils per logical block (eigen form)
+  int tile[BS_NKIND];                   // tile shape per stage: pencils per solve workgroup, else 0
   BatchStage solveT[3], hvt[3], nlT, acc;  // adjoint, indexed by call order c (substep 2 - c)
   int cap[BS_NKIND];
   int total[BS_NKIND];
@@ -3537,7 +3560,9 @@ static int batch_fwd_step(const jsd_batch* b, hipStream_t st) {
     // the solves of substeps 1 and 2 return D v of their new state (eigen form)
     int rc = (s > 0 && b->chain) ? JSD_OK : launch_stage<OpYpar>(b, b->yg, b->cap[BS_YG], st);
     if (!rc) rc = launch_stage<OpNlLin>(b, b->nl, b->cap[BS_NL], st);
-    if (!rc) rc = launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
+    if (!rc)
+      rc = b->tile[BS_SOLVE] == 8 ? launch_stage<OpSolve8>(b, b->solve8[s], b->cap[BS_SOLVE], st)
+                                  : launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
     if (rc) return rc;
   }
   return JSD_OK;
@@ -3578,9 +3603,10 @@ static float time_stage(const jsd_batch* b, const BatchStage* S, int nS, int cap
   return best;
 }
 
+// best grid of one stage kernel: one block per logical block, or persistent
+// grids of 1..8 blocks per CU; returns the time of the best
 template <class Op>
-static void tune_stage(jsd_batch* b, int kind, const BatchStage* S, int nS, hipStream_t st) {
-  // candidates: one block per logical block, or persistent grids of 1..8 blocks per CU
+static float tune_grid(const jsd_batch* b, const BatchStage* S, int nS, hipStream_t st, int* cap) {
   const int cands[5] = {S[0].total, 256 * 8, 256 * 4, 256 * 2, 256};
   int best = S[0].total;
   float tbest = 1e30f;
@@ -3592,7 +3618,32 @@ static void tune_stage(jsd_batch* b, int kind, const BatchStage* S, int nS, hipS
       best = c;
     }
   }
-  b->cap[kind] = best;
+  *cap = best;
+  return tbest;
+}
+
+template <class Op>
+static void tune_stage(jsd_batch* b, int kind, const BatchStage* S, int nS, hipStream_t st) {
+  tune_grid<Op>(b, S, nS, st, &b->cap[kind]);
+  b->tile[kind] = 0;
+}
+
+// the solve: tile shape (4 pencils per workgroup, one 16-column MFMA tile
+// per wave; or 8 pencils, two tiles per wave) and grid, timed together
+// (JSD_BATCH_TILE=4 or 8 restricts the choice: a diagnostic switch).
+static void tune_solve(jsd_batch* b, hipStream_t st) {
+  const char* env = getenv("JSD_BATCH_TILE");
+  const int only = env ? atoi(env) : 0;
+  int cap4 = 0, cap8 = 0;
+  const float t4 = only == 8 && b->chain ? 1e30f : tune_grid<OpSolve>(b, b->solve, 3, st, &cap4);
+  b->cap[BS_SOLVE] = cap4;
+  b->tile[BS_SOLVE] = 4;
+  if (!b->chain || only == 4) return;  // eight pencils per workgroup: eigen form only
+  const float t8 = tune_grid<OpSolve8>(b, b->solve8, 3, st, &cap8);
+  if (t8 < t4) {
+    b->cap[BS_SOLVE] = cap8;
+    b->tile[BS_SOLVE] = 8;
+  }
 }
 
 extern "C" int jsd_batch_destroy(jsd_batch* b) {
@@ -3688,7 +3739,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_HIP, "event creation failed");
     }
-    for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = b->total[k] = 0;
+    for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = b->total[k] = b->tile[k] = 0;
     *out = b;
     return JSD_OK;
   }
@@ -3737,7 +3788,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   {
     std::vector<YparArgs> ya;
     std::vector<NlLinArgs> na;
-    std::vector<dim3> yg, yp, ng, sg;
+    std::vector<dim3> yg, yp, ng, sg, sg8;
     size_t ylds = 0;
     for (int r = 0; r < nrun; ++r) {
       jsd_plan* p = b->plans[r];
@@ -3749,6 +3800,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       na.push_back(nl_lin_args(p, U));
       ng.push_back(dim3((unsigned)((p->fs + 255) / 256)));
       sg.push_back(dim3(solve_groups(p->EVF != nullptr, p->M, p->Z, p->Ka, p->Kb)));
+      sg8.push_back(dim3(solve_groups8(p->M, p->Z)));
     }
     BTRY(make_stage(b, b->yg, ya, yp));
     b->yg.lds = ylds;
@@ -3766,6 +3818,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
         sa.push_back(S);
       }
       BTRY(make_stage(b, b->solve[s], sa, sg));
+      if (b->chain) BTRY(make_stage(b, b->solve8[s], sa, sg8));
     }
     if (b->vjp) {
       std::vector<NlLinTArgs> nta;
@@ -3800,7 +3853,11 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   b->total[BS_HVT] = b->vjp ? b->hvt[0].total : 0;
   b->total[BS_NLT] = b->vjp ? b->nlT.total : 0;
   b->total[BS_ACC] = b->vjp ? b->acc.total : 0;
-  for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = std::max(1, b->total[k]);
+  for (int k = 0; k < BS_NKIND; ++k) {
+    b->cap[k] = std::max(1, b->total[k]);
+    b->tile[k] = 0;
+  }
+  b->tile[BS_SOLVE] = 4;
   // autotune the persistent grid of each stage on the real data, then put
   // the states (and adjoints) back as they were
   {
@@ -3829,7 +3886,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
     if (!rc) {
       tune_stage<OpYpar>(b, BS_YG, &b->yg, 1, st);
       tune_stage<OpNlLin>(b, BS_NL, &b->nl, 1, st);
-      tune_stage<OpSolve>(b, BS_SOLVE, b->solve, 3, st);
+      tune_solve(b, st);
       if (b->vjp) {
         tune_stage<OpSolveT>(b, BS_SOLVET, b->solveT, 3, st);
         tune_stage<OpYgemmHVT>(b, BS_HVT, b->hvt, 3, st);
@@ -3916,8 +3973,15 @@ extern "C" int jsd_batch_step_vjp_state(jsd_batch* b, const void* const* states,
 extern "C" int jsd_batch_info(jsd_batch* b, int* grid, int* blocks) {
   if (!b) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   for (int k = 0; k < BS_NKIND; ++k) {
-    if (grid) grid[k] = std::min(b->cap[k], b->total[k]);
-    if (blocks) blocks[k] = b->total[k];
+    const int total = (k == BS_SOLVE && b->tile[k] == 8) ? b->solve8[0].total : b->total[k];
+    if (grid) grid[k] = std::min(b->cap[k], total);
+    if (blocks) blocks[k] = total;
   }
+  return JSD_OK;
+}
+
+extern "C" int jsd_batch_tiles(jsd_batch* b, int* tile) {
+  if (!b || !tile) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  for (int k = 0; k < BS_NKIND; ++k) tile[k] = b->tile[k];
   return JSD_OK;
 }

@@ -14,7 +14,9 @@ U_t = U_yy / Re started from rest (accelerating) or from Couette flow:
 a_n = (n pi)^2 / Re, p_n = -2 Re (-1)^n / (pi n)^3, s = -1 (accelerating) or
 +1.  It is evaluated on the host at t = time_step * dt and handed to the
 HIP step (jsd_set_base_flow) before each step, the same for the step's
-three substeps (navier_stokes.py:1695).  The nonlinear term is the full
+three substeps (navier_stokes.py:1695); solve_scan steps through a device
+table of all steps' profiles built once (base_flow_table), so the loop
+never waits for the GPU.  The nonlinear term is the full
 (linearise = False) skew-symmetric perturbation term with
 coupling_term = False (:95-108): h_g gains d_z hat(U' v).  No pressure
 gradient (:83-89).
@@ -66,6 +68,24 @@ class NavierStokesVelVortPerturbationInstationary(NavierStokesVelVortPerturbatio
         y = self.physical_domain.grid_math.y
         return instationary_base_profile(y, time_step * self.get_dt(), self.Re_tau, self.kappa, self.accelerate, self.n_max)
 
+    def base_flow_table(self) -> torch.Tensor:
+        """(U, U') of every time step 0 .. n as one device table (n + 1, 2, Ny),
+        built once on the host (the same values set_base_flow derives from
+        vel_base_profile) and uploaded once: stepping through it queues a
+        device-to-device copy before each step instead of a host copy and a
+        wait (jsd_set_base_flow_device)."""
+        n = self.number_of_time_steps()
+        tab = getattr(self, "_base_table", None)
+        if tab is None or tab.shape[0] != n + 1:
+            D = self.physical_domain.grid_math.D
+            rows = []
+            for k in range(n + 1):
+                u = np.ascontiguousarray(self.vel_base_profile(k), dtype=np.float64)
+                rows.append(np.stack([u, np.ascontiguousarray(D @ u, dtype=np.float64)]))
+            tab = torch.from_numpy(np.stack(rows)).to(self._state(None).device)
+            self._base_table = tab
+        return tab
+
     def perform_time_step(
         self, vel_hat_data: Optional[torch.Tensor] = None, dPdx: Optional[float] = None, time_step: Optional[int] = None
     ) -> Tuple[torch.Tensor, float]:
@@ -82,9 +102,10 @@ class NavierStokesVelVortPerturbationInstationary(NavierStokesVelVortPerturbatio
         n = self.number_of_time_steps()
         u = self._state(None).clone()
         plan = self.plan()
+        table = self.base_flow_table()
         traj = [u.clone()]
         for k in range(n):
-            plan.set_base_flow(self.vel_base_profile(k))
+            plan.set_base_flow(table[k, 0], table[k, 1])
             plan.step(u, 0.0, 1)
             if self.write_intermediate_output or self.write_entire_output:
                 traj.append(u.clone())

@@ -9,7 +9,9 @@ time step (navier_stokes_perturbation_dual.py:591-602), with the full
 (d_x^2 + d_z^2) hat(U' v_x) (set_linearise, :103-119;
 navier_stokes_perturbation_dual.py:269-278).  No pressure gradient (:87-92).
 The forward run and its checkpointed recomputation set the forward plan's
-base flow before every forward step, as the forward equation does.
+base flow before every forward step, as the forward equation does; every
+profile comes from the forward equation's device table (base_flow_table),
+so neither direction waits for the GPU between steps.
 """
 
 from __future__ import annotations
@@ -47,11 +49,14 @@ class NavierStokesVelVortPerturbationInstationaryDual(NavierStokesVelVortPerturb
 
     def _advance_forward(self, u: torch.Tensor, k0: int, nsteps: int) -> None:
         fplan = self.forward_equation.plan()
+        table = self.forward_equation.base_flow_table()
         for k in range(k0, k0 + nsteps):
-            fplan.set_base_flow(self.forward_equation.vel_base_profile(k))
+            fplan.set_base_flow(table[k, 0], table[k, 1])
             fplan.step(u, 0.0, 1)
 
     def _dual_step(self, v: torch.Tensor, t: int, fwd_state: Optional[torch.Tensor]) -> None:
         plan = self.plan_for_step(t)
-        plan.set_base_flow(self.vel_base_profile(t))
+        table = self.forward_equation.base_flow_table()
+        k = self.number_of_time_steps - t  # the forward's step n - t
+        plan.set_base_flow(table[k, 0], table[k, 1])
         plan.dual_step(v, fwd_state)

@@ -141,9 +141,24 @@ class StepPlan:
     def shape(self) -> Tuple[int, int, int, int]:
         return (3, self.grid.nxm, self.grid.ny, self.grid.nzm)
 
-    def set_base_flow(self, base_u: np.ndarray, base_du: Optional[np.ndarray] = None) -> None:
+    def set_base_flow(self, base_u, base_du=None) -> None:
         """Replace the base profile U(y) (and U'(y), by default D U) of a
-        perturbation plan before the next step."""
+        perturbation plan before the next step.  Host arrays are copied
+        before return; float64 tensors on the plan's GPU are copied on the
+        current stream without waiting (they must stay alive until the
+        stream has run the copies)."""
+        if isinstance(base_u, torch.Tensor) and base_u.is_cuda:
+            if not (isinstance(base_du, torch.Tensor) and base_du.is_cuda):
+                raise ValueError("a device profile needs its derivative on the device too")
+            for t in (base_u, base_du):
+                if t.dtype != torch.float64 or t.shape != (self.grid.ny,) or not t.is_contiguous():
+                    raise ValueError("device base profiles need %d contiguous float64 values" % self.grid.ny)
+            _lib.check(
+                _lib.load().jsd_set_base_flow_device(
+                    self._h, ctypes.c_void_p(base_u.data_ptr()), ctypes.c_void_p(base_du.data_ptr()), _stream()
+                )
+            )
+            return
         u = np.ascontiguousarray(base_u, dtype=np.float64)
         du = np.ascontiguousarray(self.grid.D @ u if base_du is None else base_du, dtype=np.float64)
         if u.shape != (self.grid.ny,) or du.shape != (self.grid.ny,):

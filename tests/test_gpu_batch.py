@@ -71,6 +71,29 @@ def test_batch_step_equals_individual_steps():
     batch.close()
 
 
+@pytest.mark.parametrize("tile", ["4", "8"])
+def test_batch_solve_tile_shapes(monkeypatch, tile):
+    """Both tile shapes the autotuner chooses between for the batched solve
+    (4 pencils per workgroup, one MFMA column tile per wave; 8 pencils, two
+    tiles per wave), forced with JSD_BATCH_TILE, step every run as its own
+    plan does, to rounding; the batch reports the shape it runs."""
+    monkeypatch.setenv("JSD_BATCH_TILE", tile)
+    plans, u0 = _plans()
+    ind = [u.clone() for u in u0]
+    bat = [u.clone() for u in u0]
+    batch = BatchPlan(plans, bat)
+    info = batch.info()
+    assert info["solve"]["tile"] == int(tile)
+    assert all(v["tile"] == 0 for k, v in info.items() if k != "solve")
+    for p, u in zip(plans, ind):
+        p.step(u, 0.0, 3)
+    batch.step(3)
+    torch.cuda.synchronize()
+    for a, b in zip(bat, ind):
+        assert rel(a, b) <= 1e-13
+    batch.close()
+
+
 def test_batch_run_against_oracle():
     plans, u0 = _plans()
     bat = [u.clone() for u in u0]

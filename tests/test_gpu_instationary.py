@@ -167,3 +167,51 @@ def test_coupling_off_rejections():
     pert = StepPlan(g, 500.0, 0.01, KIND_PERTURBATION, linearise=True, coupling_term=False)
     with pytest.raises(JsdError):
         pert.vjp(torch.zeros(pert.shape, dtype=torch.complex128, device=DEV), 1)
+
+
+def test_device_profile_table_matches_host_profiles_and_captures():
+    """The device table of profiles (jsd_set_base_flow_device: device-to-device
+    copies, no wait) steps exactly as the host profiles of each step do, and
+    a graph holding three (set profile, step) pairs replays bit for bit,
+    with eager work between the replays."""
+    from jax_spectral_dns_amd.domain import PhysicalDomain
+    from jax_spectral_dns_amd.field import FourierField, VectorField
+    from jax_spectral_dns_amd.navier_stokes_perturbation_instationary import NavierStokesVelVortPerturbationInstationary
+
+    shape, lx, lz = (24, 33, 16), 2 * np.pi, np.pi
+    pd = PhysicalDomain.create(shape, (True, False, True), scale_factors=(lx, 1.0, lz))
+    g, _ = pair(shape, lx, lz)
+    u0 = perturbation_field(g, seed=45, energy=1e-2).to(DEV)
+    vh = VectorField.FromData(FourierField, pd, u0.clone(), "velocity_hat")
+    nse = NavierStokesVelVortPerturbationInstationary(vh, Re=500.0, dt=0.01, end_time=0.06, kappa=1.0)
+    table = nse.base_flow_table()
+    assert table.shape == (7, 2, g.ny)
+    plan = nse.plan()
+    host = u0.clone()
+    for k in range(6):
+        plan.set_base_flow(nse.vel_base_profile(k))
+        plan.step(host, 0.0, 1)
+    out, _, n = nse.solve_scan()
+    assert n == 6 and torch.equal(out[-1], host)
+    # capture steps 0..2 through the table, replay against eager steps
+    ref = u0.clone()
+    for k in range(3):
+        plan.set_base_flow(table[k, 0], table[k, 1])
+        plan.step(ref, 0.0, 1)
+    state = u0.clone()
+    for k in range(3):  # warm-up before capture
+        plan.set_base_flow(table[k, 0], table[k, 1])
+        plan.step(state, 0.0, 1)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for k in range(3):
+            plan.set_base_flow(table[k, 0], table[k, 1])
+            plan.step(state, 0.0, 1)
+    for r in range(3):
+        state.copy_(u0)
+        plan.set_base_flow(table[5, 0], table[5, 1])  # a different profile set eagerly in between
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.isfinite(torch.randn(1 << 16, device=DEV).sum().item())
+        assert torch.equal(state, ref), r
