@@ -1573,6 +1573,7 @@ __device__ void mv_mfma(const double* __restrict__ AT, int N, const double* X, d
 
 #include "jsd_solve.inc"
 #include "jsd_solve8.inc"
+#include "jsd_solve8p.inc"
 
 #include "jsd_adjoint.inc"
 #include "jsd_adjoint_nl.inc"
@@ -1676,7 +1677,9 @@ struct jsd_plan {
   double2 *twx, *twz;
   FftPlan fpx, fpz;
   int generic_fft;  // 1: run-time-length FFT kernels only (JSD_FFT_GENERIC=1, a diagnostic switch)
-  int solve8;       // eigen form: 8 pencils per solve workgroup (0 with JSD_SOLVE4=1, a diagnostic switch)
+  int solve8;       // eigen form, pencils per solve workgroup and waves: 2 (default) 8 pencils, 8 waves, one
+                    // parity per wave; 1 8 pencils, 4 waves; 0 4 pencils (JSD_SOLVE=8p / 8 / 4 or JSD_SOLVE4=1,
+                    // diagnostic switches)
   int xreg;         // x-pass kernels: 1 (default) the k_fft_x_*_r kernels, inverse inputs scattered through LDS;
                     // 2 the inverse's first stage in registers too; 0 the k_fft_x_*_t kernels (JSD_XPASS=0/1/2)
   // workspaces
@@ -2202,7 +2205,11 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     const char* env = getenv("JSD_FFT_GENERIC");
     p->generic_fft = (env && env[0] == '1') ? 1 : 0;
     const char* env4 = getenv("JSD_SOLVE4");
-    p->solve8 = (env4 && env4[0] == '1') ? 0 : 1;
+    const char* envs = getenv("JSD_SOLVE");
+    p->solve8 = 2;
+    if (envs && strcmp(envs, "8") == 0) p->solve8 = 1;
+    if ((envs && strcmp(envs, "4") == 0) || (env4 && env4[0] == '1')) p->solve8 = 0;
+    if (d->ny > 130) p->solve8 = std::min(p->solve8, 1);  // one interior row pair per thread
     const char* envx = getenv("JSD_XPASS");
     p->xreg = (envx && envx[0] >= '0' && envx[0] <= '2') ? envx[0] - '0' : 1;
   }
@@ -2867,8 +2874,15 @@ static int substep(jsd_plan* p, double2* U, const double2* UF, int s, double dPd
     S.du_full = (p->d.kind == JSD_KIND_NS) || !p->d.linearise;  // linearised steps need D v only
   }
   prof_begin(p, 5, st);
+  const bool s8p = s8 && p->solve8 == 2;
   if (!p->EVF)
     hipLaunchKernelGGL(k_solve, dim3(nblk), dim3(256), 0, st, S);
+  else if (s8p && !S.DUo)
+    hipLaunchKernelGGL(k_solve_eig8p<0>, dim3(nblk), dim3(512), 0, st, S);
+  else if (s8p && S.du_full)
+    hipLaunchKernelGGL(k_solve_eig8p<2>, dim3(nblk), dim3(512), 0, st, S);
+  else if (s8p)
+    hipLaunchKernelGGL(k_solve_eig8p<1>, dim3(nblk), dim3(512), 0, st, S);
   else if (s8 && !S.DUo)
     hipLaunchKernelGGL(k_solve_eig8<0>, dim3(nblk), dim3(256), 0, st, S);
   else if (s8 && S.du_full)

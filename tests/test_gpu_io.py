@@ -34,3 +34,28 @@ def test_solve_writes_trajectory_file(tmp_path, monkeypatch):
     VectorField([final[c] for c in range(3)], "velocity_hat").save_to_file("final")
     back = VectorField.FromFile(pd, "final", name="velocity_hat", device="cuda")
     assert torch.equal(back.get_data(), final.get_data())
+
+
+def test_dlpack_round_trip_keeps_complex128_layout():
+    """The torch half of the DLPack binding the integration notes describe:
+    a complex128 state exported and re-imported through DLPack is the same
+    buffer with the same dtype, shape and strides, and jsd_step through the
+    re-imported view equals jsd_step on the original (the JAX half is not
+    exercised: JAX is absent)."""
+    import torch.utils.dlpack as dl
+
+    from jax_spectral_dns_amd import KIND_NS
+    from jax_spectral_dns_amd.plan import StepPlan
+    from tests.helpers import ns_field, pair
+
+    g, _ = pair((24, 33, 16), 2 * np.pi, np.pi)
+    u = ns_field(g, seed=81).to("cuda")
+    v = dl.from_dlpack(dl.to_dlpack(u))
+    assert v.data_ptr() == u.data_ptr() and v.dtype == torch.complex128
+    assert v.shape == u.shape and v.stride() == u.stride()
+    plan = StepPlan(g, 3000.0, 0.005, KIND_NS)
+    a = u.clone()
+    b = dl.from_dlpack(dl.to_dlpack(u.clone()))
+    plan.step(a, -2.0 / 3000.0, 1)
+    plan.step(b, -2.0 / 3000.0, 1)
+    assert torch.equal(a, b)

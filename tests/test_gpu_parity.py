@@ -286,9 +286,10 @@ def test_multistep_call_chains_derivatives(kind):
 @pytest.mark.parametrize("kind,linearise", [(KIND_NS, False), (KIND_PERTURBATION, False), (KIND_PERTURBATION, True),
                                             (KIND_DUAL, True)])
 def test_eight_pencil_solve_matches_four_pencil_solve(monkeypatch, kind, linearise, shape):
-    """The solve with eight pencils per workgroup (the default) performs the
-    four-pencil kernel's arithmetic in the same order, with reciprocals in
-    place of two divisions: three steps agree to rounding
+    """The solves with eight pencils per workgroup (the default with eight
+    waves, one parity each; and with four waves) perform the four-pencil
+    kernel's arithmetic in the same order, with reciprocals in place of two
+    divisions: three steps agree to rounding
     (<= 1e-14 per step, measured <= 3e-15), for odd and even Ny, a last group
     that is partly empty (16 x 4 gives 17 x 5 = 85 pencils), every
     derivative-output mode and the full size."""
@@ -306,21 +307,26 @@ def test_eight_pencil_solve_matches_four_pencil_solve(monkeypatch, kind, lineari
         fwd_ops = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=True).ops
         make = lambda: StepPlan(g, 3000.0, 0.01, KIND_DUAL, linearise=True, ops=fwd_ops, explicit_dt=-0.01, plan_re=-3000.0)
     monkeypatch.delenv("JSD_SOLVE4", raising=False)
+    monkeypatch.delenv("JSD_SOLVE", raising=False)
+    p8w = make()  # default: 8 pencils, 8 waves (one parity per wave)
+    monkeypatch.setenv("JSD_SOLVE", "8")
     p8 = make()
-    monkeypatch.setenv("JSD_SOLVE4", "1")
+    monkeypatch.setenv("JSD_SOLVE", "4")
     p4 = make()
-    a, b = u.to(DEV).clone(), u.to(DEV).clone()
+    monkeypatch.delenv("JSD_SOLVE", raising=False)
+    a, b, c = u.to(DEV).clone(), u.to(DEV).clone(), u.to(DEV).clone()
     dp = -2.0 / 5000.0 if kind == KIND_NS else 0.0
     for n in range(3):
-        if kind == KIND_DUAL:
-            p8.dual_step(a, None)
-            p4.dual_step(b, None)
-        else:
-            p8.step(a, dp, 1)
-            p4.step(b, dp, 1)
+        for plan, x in ((p8, a), (p4, b), (p8w, c)):
+            if kind == KIND_DUAL:
+                plan.dual_step(x, None)
+            else:
+                plan.step(x, dp, 1)
         err = rel(a, b)
-        print("step", n, "8 vs 4 pencils rel", err)
+        err_w = rel(c, a)
+        print("step", n, "8 vs 4 pencils rel", err, "8 pencils 8 vs 4 waves rel", err_w)
         assert err <= 1e-14 * (n + 1), (n, err)
+        assert err_w <= 1e-14 * (n + 1), (n, err_w)
 
 
 # x transform lengths of the compile-time kernels: 40 (8 5), 75 (5 5 3),
