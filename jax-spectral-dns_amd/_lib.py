@@ -112,6 +112,7 @@ def load() -> ctypes.CDLL:
     lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
     lib.jsd_set_base_flow.argtypes = [vp, _dp, _dp, vp]
+    lib.jsd_set_base_flow_device.argtypes = [vp, vp, vp, vp]
     lib.jsd_step_source.argtypes = [vp, vp, vp, ctypes.c_double, vp, ctypes.c_int, vp]
     lib.jsd_substep_terms.argtypes = [vp, vp, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp, ctypes.c_int, vp]
     lib.jsd_laplacian.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, vp]
@@ -126,6 +127,7 @@ def load() -> ctypes.CDLL:
     lib.jsd_batch_step_vjp.argtypes = [vp, ctypes.c_int, vp]
     lib.jsd_batch_step_vjp_state.argtypes = [vp, ctypes.POINTER(vp), vp]
     lib.jsd_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.jsd_batch_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     lib.jsd_last_error.restype = ctypes.c_char_p
     lib.jsd_version.restype = ctypes.c_int
     for name in EXPORTS:
