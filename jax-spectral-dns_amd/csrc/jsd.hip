@@ -372,7 +372,9 @@ __host__ __device__ inline size_t ystream_lds_doubles(int KL) { return (size_t)K
 // FULL: ceil(N/2) >= 64, every tile row is a stored row (the bench sizes).
 // The loop runs whole groups of YS_PD k-steps (the ring slots are fixed
 // registers), rounded up: the extra positions read zeros and store nothing.
-template <bool ANTI, bool FULL>
+// ACC: out += the product (the adjoint's u_bar += D^T Du_bar); the centre
+// row of odd N (h = N-1-h) is added once.
+template <bool ANTI, bool FULL, bool ACC = false>
 __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
   extern __shared__ double sF[];
   const int KL = G.KL;
@@ -480,13 +482,23 @@ __global__ __launch_bounds__(YS_THREADS, 4) void k_ypar_stream(YsArgs G) {
               const double ev = ae[t][r];
               const double od = h < nOf ? ao[t][r] : 0.0;
               // the centre row (h = N-1-h) has od = 0: both stores agree
-              lp[h0 * C2] = ev + od;
-              lm[-h0 * C2] = ev - od;
+              if (ACC) {
+                lp[h0 * C2] += ev + od;
+                if (2 * h != N - 1) lm[-h0 * C2] += ev - od;
+              } else {
+                lp[h0 * C2] = ev + od;
+                lm[-h0 * C2] = ev - od;
+              }
             }
           if (G.xE && kr == 0) {
             const double od = G.xO ? po : 0.0;
-            lp[64 * C2] = pe + od;
-            lm[-64 * C2] = pe - od;
+            if (ACC) {
+              lp[64 * C2] += pe + od;
+              if (2 * 64 != N - 1) lm[-64 * C2] += pe - od;
+            } else {
+              lp[64 * C2] = pe + od;
+              lm[-64 * C2] = pe - od;
+            }
           }
         }
 #pragma unroll
@@ -1012,13 +1024,19 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
     }
     if constexpr (PZ) dft16_pz<+1>(v);
     else dft16<+1>(v);
+    // twiddles w256^(n1 k2) = w^k2 (w = w256^n1 from the table) as running
+    // powers, restarted from the table at k2 = 8: two loads and two live
+    // twiddles instead of fifteen (the powers agree with the table to ~8 ulp)
+    {
+      const double2 w1 = cconj(tw[n1]), w8 = cconj(tw[8 * n1]);
+      lds[li * REG + (t * 16 + n1) * Z16_LD] = v[0];
+      double2 wk = w1;
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) {
-      double2 y = v[k2];
-      // tw[0] = (1, -0): for n1 = 0 the product leaves finite values unchanged,
-      // and with no branch on n1 the twiddle loads issue together
-      if (k2 > 0) y = cmul(y, cconj(tw[n1 * k2]));
-      lds[li * REG + (t * 16 + n1) * Z16_LD + k2] = y;
+      for (int k2 = 1; k2 < 16; ++k2) {
+        if (k2 == 8) wk = w8;
+        lds[li * REG + (t * 16 + n1) * Z16_LD + k2] = cmul(v[k2], wk);
+        if (k2 != 7) wk = cmul(wk, w1);
+      }
     }
   }
   __syncthreads();
@@ -1071,11 +1089,16 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
   __syncthreads();
   if (threadIdx.x < 32 * LN) {
     const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, n1 = threadIdx.x & 15;
+    {  // twiddles as running powers, as in the inverse
+      const double2 w1 = tw[n1], w8 = tw[8 * n1];
+      lds[li * REG + (o * 16 + n1) * Z16_LD] = v[0];
+      double2 wk = w1;
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) {
-      double2 y = v[k2];
-      if (k2 > 0) y = cmul(y, tw[n1 * k2]);  // unconditional loads, as in the inverse
-      lds[li * REG + (o * 16 + n1) * Z16_LD + k2] = y;
+      for (int k2 = 1; k2 < 16; ++k2) {
+        if (k2 == 8) wk = w8;
+        lds[li * REG + (o * 16 + n1) * Z16_LD + k2] = cmul(v[k2], wk);
+        if (k2 != 7) wk = cmul(wk, w1);
+      }
     }
   }
   __syncthreads();
@@ -1656,6 +1679,9 @@ struct jsd_plan {
   double *YF1, *YF2;  // parity-block fragments of D and D^2 for k_ypar
   YparArgs yp1, yp2;  // their block geometry
   double *YS1, *YS2;  // the same operators in k_ypar_stream's layout
+  double* YS1T;       // D^T in that layout (adjoint plans): u_bar += D^T Du_bar
+  YsArgs ys1t;
+  int yst_grid;       // persistent grid of the accumulating k_ypar_stream (0: k_ygemm<GEMM_ACC> instead)
   double* YSZ;        // zeros read by its lanes outside the data
   YsArgs ys1, ys2;
   int ys_grid[2];     // persistent grid of k_ypar_stream for D, D^2 (0: k_ypar instead)
@@ -2143,6 +2169,36 @@ static int ypar_stream_setup(jsd_plan* p, const double* D, const double* D2) {
   return JSD_OK;
 }
 
+// D^T (centro-antisymmetric like D) for the accumulating streaming kernel of
+// the adjoint; yst_grid = 0 leaves k_ygemm<GEMM_ACC> in place.
+static int ypar_stream_setup_T(jsd_plan* p, const double* DT) {
+  p->yst_grid = 0;
+  p->YS1T = nullptr;
+  const int N = p->N;
+  if ((N + 1) / 2 > 65 || p->ys_grid[0] == 0) return JSD_OK;
+  std::vector<double> f;
+  ystream_frags(DT, N, -1, f, p->ys1t);
+  int rc = upload(p, &p->YS1T, f.data(), f.size());
+  if (rc) return rc;
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const size_t sh = ystream_lds_doubles(p->ys1t.KL) * sizeof(double);
+  const bool full = p->ys1t.nEf >= 64;
+  const void* fn = full ? (const void*)k_ypar_stream<true, true, true> : (const void*)k_ypar_stream<true, false, true>;
+  if (sh > 64 * 1024 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh) != hipSuccess) {
+    (void)hipGetLastError();
+    return JSD_OK;
+  }
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, YS_THREADS, sh) != hipSuccess) {
+    (void)hipGetLastError();
+    return JSD_OK;
+  }
+  if (nb > 0 && ncu > 0) p->yst_grid = nb * ncu;
+  return JSD_OK;
+}
+
 static void ypar_frags(const double* A, int N, int sgn, std::vector<double>& F, YparArgs& G) {
   const int nEf = (N + 1) / 2, nOf = N / 2;
   G.nEf = nEf;
@@ -2380,6 +2436,8 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   }
   p->DFT1 = p->D_R = p->D2_R = p->RB_R = p->LI_R = p->PO_R = nullptr;
   p->EH_R = p->EG_R = p->LAMR = nullptr;
+  p->YS1T = nullptr;
+  p->yst_grid = 0;
   p->SS = nullptr;
   const bool adj = (d->kind == JSD_KIND_PERTURBATION && d->linearise) ||
                    (d->adjoint && (d->kind == JSD_KIND_NS || d->kind == JSD_KIND_PERTURBATION));
@@ -2391,6 +2449,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     transpose_sq(d->D, DT.data(), N);
     tile_D(DT.data(), N, p->NT, p->KS, tiled);
     TRY(upload(p, &p->DFT1, tiled.data(), tiled.size()));
+    TRY(ypar_stream_setup_T(p, DT.data()));
     TRY(upload(p, &p->D_R, d->D, nn));
     TRY(upload(p, &p->D2_R, D2.data(), nn));
     TRY(upload(p, &p->RB_R, d->rhs_base, 3 * nn));
@@ -3243,6 +3302,35 @@ static YgemmArgs acc_args(const jsd_plan* p, double2* UB) {
   return ygemm_args(p, p->DFT1, p->DU, nullptr, nullptr, UB + p->fs, nullptr, nullptr, nullptr, 0L);
 }
 
+// out (ncomp components, component stride cstride doubles) += D^T in: the
+// accumulating streaming kernel, or k_ygemm<GEMM_ACC>
+static int launch_acc_T(jsd_plan* p, const double2* in, double2* out, int ncomp, long cstride, hipStream_t st) {
+  if (p->yst_grid > 0 && (long)ncomp * cstride < (1L << 31)) {
+    YsArgs Y = p->ys1t;
+    Y.F = p->YS1T;
+    Y.in = (const double*)in;
+    Y.out = (double*)out;
+    Y.M = p->M;
+    Y.N = p->N;
+    Y.Z = p->Z;
+    Y.comp_stride = (int)cstride;
+    Y.ncol = ncomp * p->M * 2 * p->Z;
+    Y.zero = p->YSZ;
+    const int units = (Y.ncol + 15) / 16;
+    const int grid = std::min(p->yst_grid, (units + YS_THREADS / 64 - 1) / (YS_THREADS / 64));
+    const size_t sh = ystream_lds_doubles(Y.KL) * sizeof(double);
+    if (Y.nEf >= 64)
+      hipLaunchKernelGGL((k_ypar_stream<true, true, true>), dim3(grid), dim3(YS_THREADS), sh, st, Y);
+    else
+      hipLaunchKernelGGL((k_ypar_stream<true, false, true>), dim3(grid), dim3(YS_THREADS), sh, st, Y);
+  } else {
+    hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, ygemm_grid(p, ncomp), dim3(256), 0, st,
+                       ygemm_args(p, p->DFT1, in, nullptr, nullptr, out, nullptr, nullptr, nullptr, cstride));
+  }
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
+}
+
 static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
   const Roles r = roles_of(p);
   const int nblk = ((solve_groups(p->EH_R != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
@@ -3253,8 +3341,8 @@ static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipS
   // state adjoint += transposed linearised product term; Dvbar -> DU[0]
   hipLaunchKernelGGL(k_nl_lin_T, dim3((p->fs + 255) / 256), dim3(256), 0, st, nl_lin_T_args(p, UB));
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, ygemm_grid(p, 1), dim3(256), 0, st, acc_args(p, UB));
-  HIPCHK(hipGetLastError());
+  const int rc = launch_acc_T(p, p->DU, UB + p->fs, 1, 0L, st);  // vbar += D^T Dvbar
+  if (rc) return rc;
   // carries of this substep's old-term usage feed the previous substep
   std::swap(p->HV, p->HVo);
   std::swap(p->HG, p->HGo);
@@ -3345,10 +3433,7 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStre
     HIPCHK(hipGetLastError());
   }
   // (iv) ubar_c += D^T Dubar_c
-  hipLaunchKernelGGL(k_ygemm<GEMM_ACC>, ygemm_grid(p, 3), dim3(256), 0, st,
-                     ygemm_args(p, p->DFT1, p->DU, nullptr, nullptr, UB, nullptr, nullptr, nullptr, (long)(2 * fs)));
-  HIPCHK(hipGetLastError());
-  return JSD_OK;
+  return launch_acc_T(p, p->DU, UB, 3, (long)(2 * fs), st);
 }
 
 static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s, bool has_carry, hipStream_t st) {
