@@ -1600,6 +1600,7 @@ __device__ void mv_mfma(const double* __restrict__ AT, int N, const double* X, d
 
 #include "jsd_adjoint.inc"
 #include "jsd_adjoint_nl.inc"
+#include "jsd_adjoint8p.inc"
 
 // ===========================================================================
 // energy: E = sum_c sum_l (w_l / 4) s^2/(Px Pz) sum_k |Herm(u_c)|^2
@@ -1680,6 +1681,9 @@ struct jsd_plan {
   YparArgs yp1, yp2;  // their block geometry
   double *YS1, *YS2;  // the same operators in k_ypar_stream's layout
   double* YS1T;       // D^T in that layout (adjoint plans): u_bar += D^T Du_bar
+  double* TF;         // eigen-form adjoint plans: transposed solve blocks (k_solve_T8p)
+  int solveT8;        // 1: k_solve_T8p (default when TF is set); 0: k_solve_T + the h_v / h_g transpose
+                      // (JSD_SOLVET=table, a diagnostic switch)
   YsArgs ys1t;
   int yst_grid;       // persistent grid of the accumulating k_ypar_stream (0: k_ygemm<GEMM_ACC> instead)
   double* YSZ;        // zeros read by its lanes outside the data
@@ -2038,6 +2042,31 @@ static int eigen_tables(jsd_plan* p, const jsd_plan_desc* d, int N, size_t nk, c
   if (!rc) rc = upload(p, &p->GF, gf.data(), gf.size());
   if (!rc) rc = upload(p, &p->CWE, cwe.data(), cwe.size());
   if (!rc) rc = upload(p, &p->RWE, rwe.data(), rwe.size());
+  // the transposed blocks of the eigen-form adjoint solve (k_solve_T8p):
+  // [E^T][(D E)^T][E^-T][(E^-1 D)^T], each for output parity e, o; the
+  // parity-swapping blocks transpose the block of the other output parity
+  p->TF = nullptr;
+  const bool adj = (d->kind == JSD_KIND_PERTURBATION && d->linearise) ||
+                   (d->adjoint && (d->kind == JSD_KIND_NS || d->kind == JSD_KIND_PERTURBATION));
+  if (!rc && adj) {
+    std::vector<double> tf(8 * (size_t)PFRAG), dense((size_t)PH * PH), tr((size_t)PH * PH);
+    auto tile_T = [&](const double* fr, double* out) {
+      for (int t = 0; t < 4; ++t)
+        for (int ks = 0; ks < 16; ++ks)
+          for (int lane = 0; lane < 64; ++lane)
+            dense[(size_t)(t * 16 + (lane & 15)) * PH + ks * 4 + (lane >> 4)] = fr[(t * 16 + ks) * 64 + lane];
+      for (int i = 0; i < PH; ++i)
+        for (int j = 0; j < PH; ++j) tr[(size_t)i * PH + j] = dense[(size_t)j * PH + i];
+      parity_tile(tr.data(), out);
+    };
+    for (int q = 0; q < 2; ++q) {
+      tile_T(frag.data() + q * PFRAG, tf.data() + (0 * 2 + q) * PFRAG);
+      tile_T(gf.data() + (1 - q) * PFRAG, tf.data() + (1 * 2 + q) * PFRAG);
+      tile_T(ifrag.data() + q * PFRAG, tf.data() + (2 * 2 + q) * PFRAG);
+      tile_T(ff.data() + (1 - q) * PFRAG, tf.data() + (3 * 2 + q) * PFRAG);
+    }
+    rc = upload(p, &p->TF, tf.data(), tf.size());
+  }
   return rc;
 }
 
@@ -2257,9 +2286,13 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   if (d->kind != JSD_KIND_NS && (!d->base_u || !d->base_du)) JSD_FAIL(JSD_ERR_ARG, "missing base-flow profile");
   jsd_plan* p = new jsd_plan();
   p->d = *d;
+  p->TF = nullptr;
   {
     const char* env = getenv("JSD_FFT_GENERIC");
     p->generic_fft = (env && env[0] == '1') ? 1 : 0;
+    const char* envt = getenv("JSD_SOLVET");
+    p->solveT8 = (envt && strcmp(envt, "table") == 0) ? 0 : 1;
+    if (d->ny > 130) p->solveT8 = 0;
     const char* env4 = getenv("JSD_SOLVE4");
     const char* envs = getenv("JSD_SOLVE");
     p->solve8 = 2;
@@ -3292,6 +3325,45 @@ static SolveTArgs solve_T_args(const jsd_plan* p, double2* UB, int s, bool has_c
   S.has_carry = has_carry ? 1 : 0;
   return S;
 }
+static SolveT8Args solve_T8_args(const jsd_plan* p, double2* UB, int s, bool has_carry, const Roles& r) {
+  const size_t nn = (size_t)p->N * p->N;
+  const size_t nk = (size_t)p->Ka * p->Kb;
+  SolveT8Args S;
+  S.UB = UB;
+  S.cs = (long)p->fs;
+  S.HB = p->H;
+  S.CVI = r.HV;
+  S.CGI = r.HG;
+  S.CVO = r.HVo;
+  S.CGO = r.HGo;
+  S.CB = r.CONV;
+  S.CBO = r.CONVo;
+  S.TF = p->TF;
+  S.LAM = p->LAM;
+  S.CWE = p->CWE;
+  S.RWE = p->RWE;
+  S.VAP = p->VAP + (size_t)s * nk * 4 * PH;
+  S.VINV = p->VINV + (size_t)s * nk * 2;
+  S.RNS = p->RNS + (size_t)s * 4 * nn;
+  S.LNS = p->LNS + (size_t)s * 4 * nn;
+  S.kx = p->kx;
+  S.kz = p->kz;
+  S.M = p->M;
+  S.N = p->N;
+  S.Z = p->Z;
+  S.Kb = p->Kb;
+  S.nE = p->nE;
+  S.nO = p->nO;
+  const double dt = p->d.dt;
+  S.c_impl = p->c_impl[s];
+  S.diag_alpha = RK_ALP[s] * fabs(dt) / fabs(p->d.re_tau);
+  S.dt_gamma = dt * RK_GAM[s];
+  S.dt_xi = dt * RK_XI[s];
+  S.use_old = s > 0;
+  S.has_carry = has_carry ? 1 : 0;
+  return S;
+}
+
 // Hbar from (hvbar, hgbar, convbar): transposed h_v / h_g assembly
 static YgemmArgs hvt_args(const jsd_plan* p, const Roles& r) {
   double* hb = (double*)p->H;
@@ -3300,6 +3372,24 @@ static YgemmArgs hvt_args(const jsd_plan* p, const Roles& r) {
 // vbar += D^T Dvbar
 static YgemmArgs acc_args(const jsd_plan* p, double2* UB) {
   return ygemm_args(p, p->DFT1, p->DU, nullptr, nullptr, UB + p->fs, nullptr, nullptr, nullptr, 0L);
+}
+
+// The solve transpose of one substep and the h_v / h_g transpose: one
+// eigen-form kernel writing Hbar (p->H), or the table form and k_ygemm<GEMM_HVT>.
+static int solve_T_stage(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
+  const Roles r = roles_of(p);
+  if (p->TF && p->solveT8) {
+    const int nblk = ((solve_groups8(p->M, p->Z) + 7) / 8) * 8;
+    hipLaunchKernelGGL(k_solve_T8p, dim3(nblk), dim3(512), 0, st, solve_T8_args(p, UB, s, has_carry, r));
+    HIPCHK(hipGetLastError());
+    return JSD_OK;
+  }
+  const int nblk = ((solve_groups(p->EH_R != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));
+  HIPCHK(hipGetLastError());
+  return JSD_OK;
 }
 
 // out (ncomp components, component stride cstride doubles) += D^T in: the
@@ -3332,12 +3422,8 @@ static int launch_acc_T(jsd_plan* p, const double2* in, double2* out, int ncomp,
 }
 
 static int adjoint_substep(jsd_plan* p, double2* UB, int s, bool has_carry, hipStream_t st) {
-  const Roles r = roles_of(p);
-  const int nblk = ((solve_groups(p->EH_R != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
-  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));
-  HIPCHK(hipGetLastError());
+  int rc0 = solve_T_stage(p, UB, s, has_carry, st);
+  if (rc0) return rc0;
   // state adjoint += transposed linearised product term; Dvbar -> DU[0]
   hipLaunchKernelGGL(k_nl_lin_T, dim3((p->fs + 255) / 256), dim3(256), 0, st, nl_lin_T_args(p, UB));
   HIPCHK(hipGetLastError());
@@ -3437,13 +3523,9 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStre
 }
 
 static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s, bool has_carry, hipStream_t st) {
-  const Roles r = roles_of(p);
-  const int nblk = ((solve_groups(p->EH_R != nullptr, p->M, p->Z, p->Ka, p->Kb) + 7) / 8) * 8;
-  hipLaunchKernelGGL(k_solve_T, dim3(nblk), dim3(256), 0, st, solve_T_args(p, UB, s, has_carry, r));
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_ygemm<GEMM_HVT>, ygemm_grid(p, 1), dim3(256), 0, st, hvt_args(p, r));
-  HIPCHK(hipGetLastError());
-  int rc = nl_adjoint_terms(p, UB, Us, st);
+  int rc = solve_T_stage(p, UB, s, has_carry, st);
+  if (rc) return rc;
+  rc = nl_adjoint_terms(p, UB, Us, st);
   if (rc) return rc;
   std::swap(p->HV, p->HVo);
   std::swap(p->HG, p->HGo);

@@ -171,3 +171,54 @@ def test_parameter_gradient_against_oracle_autograd():
         err = rel(a, torch.view_as_complex(b))
         print(name, "grad rel", err)
         assert err < 1e-9, name
+
+
+@pytest.mark.parametrize("shape", [(24, 65, 4), (24, 48, 16), (24, 33, 16)])
+def test_eigen_adjoint_solve_matches_table_adjoint(monkeypatch, shape):
+    """The eigen-form transpose of the solve with the h_v / h_g transpose folded
+    in (k_solve_T8p, the default) against the table-form transpose and
+    k_ygemm<GEMM_HVT> (JSD_SOLVET=table): the same adjoint to rounding over 1
+    and 3 steps, odd and even Ny, linearised and nonlinear; and the
+    transposition identity of the eigen form holds also for inputs with
+    nonzero wall values, which enter the forward step through the wall
+    columns of D and D^2."""
+    from jax_spectral_dns_amd import KIND_NS
+
+    g, _ = pair(shape, 2 * np.pi, np.pi)
+    rng = np.random.default_rng(5)
+
+    def rnd():
+        return torch.from_numpy(rng.standard_normal((3, g.nxm, g.ny, g.nzm)) + 1j * rng.standard_normal((3, g.nxm, g.ny, g.nzm))).to(DEV)
+
+    monkeypatch.delenv("JSD_SOLVET", raising=False)
+    pe = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=True)
+    monkeypatch.setenv("JSD_SOLVET", "table")
+    pt = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=True)
+    monkeypatch.delenv("JSD_SOLVET", raising=False)
+    x = perturbation_field(g, seed=23, energy=1.0).to(DEV) + 0.1 * rnd()
+    y = rnd()
+    for n in (1, 3):
+        a, b = y.clone(), y.clone()
+        pe.vjp(a, n)
+        pt.vjp(b, n)
+        err = rel(a, b)
+        print("linearised, n=%d: eigen vs table adjoint rel %.3e" % (n, err))
+        assert err < 1e-12 * n
+        ax = x.clone()
+        pe.step(ax, 0.0, n)
+        lhs, rhs = _dot(ax, y), _dot(x, a)
+        print("  <Ax,y>=%.15e <x,A^T y>=%.15e (walls of x nonzero)" % (lhs, rhs))
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs) + 1e-13 * float(ax.abs().max() * y.abs().max())
+    # nonlinear NS step, adjoint about a forward state
+    u = perturbation_field(g, seed=24, energy=1e-2).to(DEV)
+    monkeypatch.delenv("JSD_SOLVET", raising=False)
+    ne = StepPlan(g, 3000.0, 0.005, KIND_NS, adjoint=True)
+    monkeypatch.setenv("JSD_SOLVET", "table")
+    nt = StepPlan(g, 3000.0, 0.005, KIND_NS, adjoint=True)
+    monkeypatch.delenv("JSD_SOLVET", raising=False)
+    a, b = y.clone(), y.clone()
+    ne.vjp_state(u, a, -2.0 / 3000.0)
+    nt.vjp_state(u, b, -2.0 / 3000.0)
+    err = rel(a, b)
+    print("NS: eigen vs table adjoint rel %.3e" % err)
+    assert err < 1e-11
