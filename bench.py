@@ -244,7 +244,7 @@ def batch_bench(nsteps: int) -> dict:
         "max_rel_diff_gain": jdiff,
         "autotuned_grid": {k: v["grid"] for k, v in ginfo.items()},
         "logical_blocks": {k: v["blocks"] for k, v in ginfo.items()},
-        "autotuned_tile": {k: v["tile"] for k, v in ginfo.items() if v["tile"]},
+        "autotuned_tile": {k: {"pencils": v["tile"], "waves": v["waves"]} for k, v in ginfo.items() if v["tile"]},
         "plan_build_s": round(t_build, 2),
         "autotune_and_capture_s": round(t_tune, 2),
     }

@@ -267,13 +267,16 @@ int jsd_batch_step_vjp_state(jsd_batch* batch, const void* const* states, void* 
  * transpose, D^T accumulation) and their total logical block counts (all 0
  * for a batch of nonlinear runs, which launches each run's own kernels). */
 int jsd_batch_info(jsd_batch* batch, int* grid, int* blocks);
-/* Tile shapes the autotuner chose, per stage as in jsd_batch_info: for the
- * forward solve the pencils per workgroup (4: one 16-column MFMA tile per
- * wave; 8: two tiles per wave, eigen-form plans only), timed together with
- * the grid size on the batch's own data; 0 for stages with one shape and
- * for batches of nonlinear runs.  Replaces nothing in the reference (which
- * has no batching); it reports how the batch runs. */
-int jsd_batch_tiles(jsd_batch* batch, int* tile);
+/* Tile shapes, per stage as in jsd_batch_info: pencils and waves per
+ * workgroup.  The autotuner times the forward solve's shapes together with
+ * the grid size on the batch's own data: 4 pencils on 4 waves (one 16-column
+ * MFMA tile per wave), 8 on 4 waves (two tiles per wave) or 8 on 8 waves
+ * (one parity per wave); the last two for eigen-form plans only.  The
+ * adjoint solve reports 8 / 8 (its eigen-form transpose) or 4 / 4 (table
+ * form).  0 for stages with one shape and for batches of nonlinear runs.
+ * waves may be NULL.  Replaces nothing in the reference (which has no
+ * batching); it reports how the batch runs. */
+int jsd_batch_tiles(jsd_batch* batch, int* pencils, int* waves);
 
 const char* jsd_last_error(void);
 int jsd_version(void);

@@ -127,7 +127,7 @@ def load() -> ctypes.CDLL:
     lib.jsd_batch_step_vjp.argtypes = [vp, ctypes.c_int, vp]
     lib.jsd_batch_step_vjp_state.argtypes = [vp, ctypes.POINTER(vp), vp]
     lib.jsd_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
-    lib.jsd_batch_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    lib.jsd_batch_tiles.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.jsd_last_error.restype = ctypes.c_char_p
     lib.jsd_version.restype = ctypes.c_int
     for name in EXPORTS:

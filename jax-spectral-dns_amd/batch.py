@@ -110,9 +110,10 @@ class BatchPlan:
         g = (ctypes.c_int * len(STAGES))()
         b = (ctypes.c_int * len(STAGES))()
         t = (ctypes.c_int * len(STAGES))()
+        w = (ctypes.c_int * len(STAGES))()
         _lib.check(_lib.load().jsd_batch_info(self._h, g, b))
-        _lib.check(_lib.load().jsd_batch_tiles(self._h, t))
-        return {s: {"grid": g[i], "blocks": b[i], "tile": t[i]} for i, s in enumerate(STAGES)}
+        _lib.check(_lib.load().jsd_batch_tiles(self._h, t, w))
+        return {s: {"grid": g[i], "blocks": b[i], "tile": t[i], "waves": w[i]} for i, s in enumerate(STAGES)}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

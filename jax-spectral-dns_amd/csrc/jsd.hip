@@ -3602,14 +3602,17 @@ extern "C" int jsd_energy_grad(jsd_plan* p, const void* vel_hat, void* out, doub
 // persistent grid of any size.
 
 struct OpYgemmHVT {
+  static constexpr int THREADS = 256;
   typedef YgemmArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_HVT>(a, k); }
 };
 struct OpYgemmAcc {
+  static constexpr int THREADS = 256;
   typedef YgemmArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { ygemm_body<GEMM_ACC>(a, k); }
 };
 struct OpYpar {
+  static constexpr int THREADS = 256;
   typedef YparArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) {
     __shared__ double sO[YP_TMAX * 16 * 32];
@@ -3617,14 +3620,17 @@ struct OpYpar {
   }
 };
 struct OpNlLin {
+  static constexpr int THREADS = 256;
   typedef NlLinArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { nl_lin_body(a, k); }
 };
 struct OpNlLinT {
+  static constexpr int THREADS = 256;
   typedef NlLinTArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { nl_lin_T_body(a, k); }
 };
 struct OpSolve {
+  static constexpr int THREADS = 256;
   typedef SolveArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) {
     if (!a.EVF)
@@ -3638,6 +3644,7 @@ struct OpSolve {
 // the same solve with eight pencils per logical block (two column tiles per
 // wave); eigen form only
 struct OpSolve8 {
+  static constexpr int THREADS = 256;
   typedef SolveArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) {
     if (!a.DUo)
@@ -3647,13 +3654,31 @@ struct OpSolve8 {
   }
 };
 struct OpSolveT {
+  static constexpr int THREADS = 256;
   typedef SolveTArgs Args;
   static __device__ __forceinline__ void run(const Args& a, Blk k) { solve_T_body(a, k.x); }
+};
+// eight pencils on eight waves, one parity per wave (eigen form)
+struct OpSolve8p {
+  static constexpr int THREADS = 512;
+  typedef SolveArgs Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) {
+    if (!a.DUo)
+      solve_body_eig8p<0>(a, k.x);
+    else
+      solve_body_eig8p<1>(a, k.x);
+  }
+};
+// the eigen-form transpose of the solve with the h_v / h_g transpose folded in
+struct OpSolveT8p {
+  static constexpr int THREADS = 512;
+  typedef SolveT8Args Args;
+  static __device__ __forceinline__ void run(const Args& a, Blk k) { solve_T_body8p(a, k.x); }
 };
 
 // runs[r] = (nx, ny, nz, first logical block of run r), ordered by first
 template <class Op>
-__global__ __launch_bounds__(256) void k_batched(const typename Op::Args* __restrict__ args,
+__global__ __launch_bounds__(Op::THREADS) void k_batched(const typename Op::Args* __restrict__ args,
                                                  const int4* __restrict__ runs, int nrun, int total) {
   for (int b = blockIdx.x; b < total; b += gridDim.x) {
     int lo = 0, hi = nrun - 1;
@@ -3687,7 +3712,10 @@ struct jsd_batch {
   std::vector<void*> allocs;
   BatchStage yg, nl, solve[3];          // forward: y-GEMM (D v), linearised term, solve per substep
   BatchStage solve8[3];                 // the solve with eight pencils per logical block (eigen form)
-  int tile[BS_NKIND];                   // tile shape per stage: pencils per solve workgroup, else 0
+  int tile[BS_NKIND];                   // tile shape per stage: pencils per solve workgroup (4, 8), else 0
+  int waves[BS_NKIND];                  // and waves per workgroup (4, 8)
+  bool adj8 = false;                    // adjoint: eigen-form transpose with the h_v / h_g transpose folded in
+  BatchStage solveT8[3];
   BatchStage solveT[3], hvt[3], nlT, acc;  // adjoint, indexed by call order c (substep 2 - c)
   int cap[BS_NKIND];
   int total[BS_NKIND];
@@ -3730,8 +3758,8 @@ static int make_stage(jsd_batch* b, BatchStage& S, const std::vector<A>& args, c
 template <class Op>
 static int launch_stage(const jsd_batch* b, const BatchStage& S, int cap, hipStream_t st) {
   const int grid = std::max(1, std::min(S.total, cap));
-  hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(256), S.lds, st, (const typename Op::Args*)S.args, S.runs,
-                     b->nrun, S.total);
+  hipLaunchKernelGGL(k_batched<Op>, dim3(grid), dim3(Op::THREADS), S.lds, st, (const typename Op::Args*)S.args,
+                     S.runs, b->nrun, S.total);
   HIPCHK(hipGetLastError());
   return JSD_OK;
 }
@@ -3741,9 +3769,14 @@ static int batch_fwd_step(const jsd_batch* b, hipStream_t st) {
     // the solves of substeps 1 and 2 return D v of their new state (eigen form)
     int rc = (s > 0 && b->chain) ? JSD_OK : launch_stage<OpYpar>(b, b->yg, b->cap[BS_YG], st);
     if (!rc) rc = launch_stage<OpNlLin>(b, b->nl, b->cap[BS_NL], st);
-    if (!rc)
-      rc = b->tile[BS_SOLVE] == 8 ? launch_stage<OpSolve8>(b, b->solve8[s], b->cap[BS_SOLVE], st)
-                                  : launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
+    if (!rc) {
+      if (b->tile[BS_SOLVE] == 8 && b->waves[BS_SOLVE] == 8)
+        rc = launch_stage<OpSolve8p>(b, b->solve8[s], b->cap[BS_SOLVE], st);
+      else if (b->tile[BS_SOLVE] == 8)
+        rc = launch_stage<OpSolve8>(b, b->solve8[s], b->cap[BS_SOLVE], st);
+      else
+        rc = launch_stage<OpSolve>(b, b->solve[s], b->cap[BS_SOLVE], st);
+    }
     if (rc) return rc;
   }
   return JSD_OK;
@@ -3751,8 +3784,13 @@ static int batch_fwd_step(const jsd_batch* b, hipStream_t st) {
 
 static int batch_bwd_step(const jsd_batch* b, hipStream_t st) {
   for (int c = 0; c < 3; ++c) {
-    int rc = launch_stage<OpSolveT>(b, b->solveT[c], b->cap[BS_SOLVET], st);
-    if (!rc) rc = launch_stage<OpYgemmHVT>(b, b->hvt[c], b->cap[BS_HVT], st);
+    int rc;
+    if (b->adj8) {
+      rc = launch_stage<OpSolveT8p>(b, b->solveT8[c], b->cap[BS_SOLVET], st);
+    } else {
+      rc = launch_stage<OpSolveT>(b, b->solveT[c], b->cap[BS_SOLVET], st);
+      if (!rc) rc = launch_stage<OpYgemmHVT>(b, b->hvt[c], b->cap[BS_HVT], st);
+    }
     if (!rc) rc = launch_stage<OpNlLinT>(b, b->nlT, b->cap[BS_NLT], st);
     if (!rc) rc = launch_stage<OpYgemmAcc>(b, b->acc, b->cap[BS_ACC], st);
     if (rc) return rc;
@@ -3813,17 +3851,28 @@ static void tune_stage(jsd_batch* b, int kind, const BatchStage* S, int nS, hipS
 // per wave; or 8 pencils, two tiles per wave) and grid, timed together
 // (JSD_BATCH_TILE=4 or 8 restricts the choice: a diagnostic switch).
 static void tune_solve(jsd_batch* b, hipStream_t st) {
-  const char* env = getenv("JSD_BATCH_TILE");
-  const int only = env ? atoi(env) : 0;
+  const char* env = getenv("JSD_BATCH_TILE");  // "4", "8" (four waves) or "8p" (eight waves)
+  const int only = env ? (strcmp(env, "8p") == 0 ? 88 : atoi(env)) : 0;
   int cap4 = 0, cap8 = 0;
-  const float t4 = only == 8 && b->chain ? 1e30f : tune_grid<OpSolve>(b, b->solve, 3, st, &cap4);
+  const float t4 = (only == 8 || only == 88) && b->chain ? 1e30f : tune_grid<OpSolve>(b, b->solve, 3, st, &cap4);
   b->cap[BS_SOLVE] = cap4;
   b->tile[BS_SOLVE] = 4;
+  b->waves[BS_SOLVE] = 4;
   if (!b->chain || only == 4) return;  // eight pencils per workgroup: eigen form only
-  const float t8 = tune_grid<OpSolve8>(b, b->solve8, 3, st, &cap8);
-  if (t8 < t4) {
+  const float t8 = only == 88 ? 1e30f : tune_grid<OpSolve8>(b, b->solve8, 3, st, &cap8);
+  float tbest = t4;
+  if (t8 < tbest) {
+    tbest = t8;
     b->cap[BS_SOLVE] = cap8;
     b->tile[BS_SOLVE] = 8;
+  }
+  if (only == 8) return;
+  int cap8p = 0;
+  const float t8p = tune_grid<OpSolve8p>(b, b->solve8, 3, st, &cap8p);
+  if (t8p < tbest) {
+    b->cap[BS_SOLVE] = cap8p;
+    b->tile[BS_SOLVE] = 8;
+    b->waves[BS_SOLVE] = 8;
   }
 }
 
@@ -3920,7 +3969,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       jsd_batch_destroy(b);
       JSD_FAIL(JSD_ERR_HIP, "event creation failed");
     }
-    for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = b->total[k] = b->tile[k] = 0;
+    for (int k = 0; k < BS_NKIND; ++k) b->cap[k] = b->total[k] = b->tile[k] = b->waves[k] = 0;
     *out = b;
     return JSD_OK;
   }
@@ -3956,6 +4005,8 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   }
   b->chain = true;
   for (int r = 0; r < nrun; ++r) b->chain = b->chain && plans[r]->EVF != nullptr;
+  b->adj8 = b->vjp && b->chain;
+  for (int r = 0; r < nrun; ++r) b->adj8 = b->adj8 && plans[r]->TF != nullptr && plans[r]->solveT8;
 #define BTRY(x)               \
   do {                        \
     int _rc = (x);            \
@@ -4022,23 +4073,38 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
           ta.push_back(solve_T_args(p, b->adj[r], s, c > 0, rr));
           ha.push_back(hvt_args(p, rr));
         }
-        BTRY(make_stage(b, b->solveT[c], ta, sg));
-        BTRY(make_stage(b, b->hvt[c], ha, yg));
+        if (b->adj8) {
+          std::vector<SolveT8Args> t8;
+          for (int r = 0; r < nrun; ++r) {
+            jsd_plan* p = b->plans[r];
+            const Roles r0 = roles_of(p);
+            t8.push_back(solve_T8_args(p, b->adj[r], s, c > 0, (c & 1) ? roles_swapped(r0) : r0));
+          }
+          BTRY(make_stage(b, b->solveT8[c], t8, sg8));
+        } else {
+          BTRY(make_stage(b, b->solveT[c], ta, sg));
+          BTRY(make_stage(b, b->hvt[c], ha, yg));
+        }
       }
     }
   }
   b->total[BS_YG] = b->yg.total;
   b->total[BS_NL] = b->nl.total;
   b->total[BS_SOLVE] = b->solve[0].total;
-  b->total[BS_SOLVET] = b->vjp ? b->solveT[0].total : 0;
-  b->total[BS_HVT] = b->vjp ? b->hvt[0].total : 0;
+  b->total[BS_SOLVET] = b->vjp ? (b->adj8 ? b->solveT8[0].total : b->solveT[0].total) : 0;
+  b->total[BS_HVT] = b->vjp && !b->adj8 ? b->hvt[0].total : 0;
   b->total[BS_NLT] = b->vjp ? b->nlT.total : 0;
   b->total[BS_ACC] = b->vjp ? b->acc.total : 0;
   for (int k = 0; k < BS_NKIND; ++k) {
     b->cap[k] = std::max(1, b->total[k]);
-    b->tile[k] = 0;
+    b->tile[k] = b->waves[k] = 0;
   }
   b->tile[BS_SOLVE] = 4;
+  b->waves[BS_SOLVE] = 4;
+  if (b->vjp) {
+    b->tile[BS_SOLVET] = b->adj8 ? 8 : 4;
+    b->waves[BS_SOLVET] = b->adj8 ? 8 : 4;
+  }
   // autotune the persistent grid of each stage on the real data, then put
   // the states (and adjoints) back as they were
   {
@@ -4069,8 +4135,12 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       tune_stage<OpNlLin>(b, BS_NL, &b->nl, 1, st);
       tune_solve(b, st);
       if (b->vjp) {
-        tune_stage<OpSolveT>(b, BS_SOLVET, b->solveT, 3, st);
-        tune_stage<OpYgemmHVT>(b, BS_HVT, b->hvt, 3, st);
+        if (b->adj8) {
+          tune_grid<OpSolveT8p>(b, b->solveT8, 3, st, &b->cap[BS_SOLVET]);
+        } else {
+          tune_grid<OpSolveT>(b, b->solveT, 3, st, &b->cap[BS_SOLVET]);
+          tune_stage<OpYgemmHVT>(b, BS_HVT, b->hvt, 3, st);
+        }
         tune_stage<OpNlLinT>(b, BS_NLT, &b->nlT, 1, st);
         tune_stage<OpYgemmAcc>(b, BS_ACC, &b->acc, 1, st);
       }
@@ -4154,15 +4224,19 @@ extern "C" int jsd_batch_step_vjp_state(jsd_batch* b, const void* const* states,
 extern "C" int jsd_batch_info(jsd_batch* b, int* grid, int* blocks) {
   if (!b) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   for (int k = 0; k < BS_NKIND; ++k) {
-    const int total = (k == BS_SOLVE && b->tile[k] == 8) ? b->solve8[0].total : b->total[k];
+    const int total = (k == BS_SOLVE && b->tile[k] == 8) ? b->solve8[0].total : b->total[k];  // BS_SOLVET: total
+                                                                                              // is of its kernel
     if (grid) grid[k] = std::min(b->cap[k], total);
     if (blocks) blocks[k] = total;
   }
   return JSD_OK;
 }
 
-extern "C" int jsd_batch_tiles(jsd_batch* b, int* tile) {
-  if (!b || !tile) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
-  for (int k = 0; k < BS_NKIND; ++k) tile[k] = b->tile[k];
+extern "C" int jsd_batch_tiles(jsd_batch* b, int* pencils, int* waves) {
+  if (!b || !pencils) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  for (int k = 0; k < BS_NKIND; ++k) {
+    pencils[k] = b->tile[k];
+    if (waves) waves[k] = b->waves[k];
+  }
   return JSD_OK;
 }

@@ -71,19 +71,21 @@ def test_batch_step_equals_individual_steps():
     batch.close()
 
 
-@pytest.mark.parametrize("tile", ["4", "8"])
+@pytest.mark.parametrize("tile", ["4", "8", "8p"])
 def test_batch_solve_tile_shapes(monkeypatch, tile):
-    """Both tile shapes the autotuner chooses between for the batched solve
-    (4 pencils per workgroup, one MFMA column tile per wave; 8 pencils, two
-    tiles per wave), forced with JSD_BATCH_TILE, step every run as its own
-    plan does, to rounding; the batch reports the shape it runs."""
+    """The tile shapes the autotuner chooses between for the batched solve
+    (4 pencils on 4 waves, one MFMA column tile per wave; 8 pencils on 4
+    waves, two tiles per wave; 8 pencils on 8 waves, one parity per wave),
+    forced with JSD_BATCH_TILE, step every run as its own plan does, to
+    rounding; the batch reports the shape it runs."""
     monkeypatch.setenv("JSD_BATCH_TILE", tile)
     plans, u0 = _plans()
     ind = [u.clone() for u in u0]
     bat = [u.clone() for u in u0]
     batch = BatchPlan(plans, bat)
     info = batch.info()
-    assert info["solve"]["tile"] == int(tile)
+    assert info["solve"]["tile"] == (4 if tile == "4" else 8)
+    assert info["solve"]["waves"] == (8 if tile == "8p" else 4)
     assert all(v["tile"] == 0 for k, v in info.items() if k != "solve")
     for p, u in zip(plans, ind):
         p.step(u, 0.0, 3)
