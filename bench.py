@@ -34,12 +34,12 @@ FP64_MFMA_PEAK_TFS = 78.6
 TRAFFIC_FILE = "hbm_traffic_ns_128x129x128.json"
 CLASSES = ("ygemm", "fft_x_inv", "fft_z_mid", "fft_x_fwd", "ygemm_hv", "solve", "nl_lin")
 KERNEL_NAMES = {
-    "ygemm": "k_ypar_stream<true, true>",
-    "fft_x_inv": "k_fft_x_inv_t<Rad<8, 5, 5, 1> >",
+    "ygemm": "k_ypar_stream<true, true, false>",
+    "fft_x_inv": "k_fft_x_inv_r<Rad<8, 5, 5, 1>, false>",
     "fft_z_mid": "k_fft_z_mid_256<6, 0, true, 2>",
-    "fft_x_fwd": "k_fft_x_fwd_t<Rad<8, 5, 5, 1> >",
+    "fft_x_fwd": "k_fft_x_fwd_r<Rad<8, 5, 5, 1> >",
     "ygemm_hv": "k_ygemm<1>",
-    "solve": "k_solve_eig8",
+    "solve": "k_solve_eig8p",
     "nl_lin": "k_nl_lin",
 }
 
@@ -334,6 +334,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-runs figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the nonlinear-gradient figure")
     ap.add_argument("--adjoint-steps", type=int, default=40)
+    ap.add_argument("--adjoint-runs", type=int, default=8, help="runs of the nonlinear-gradient batch")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument(
         "--copy-probe",
@@ -366,7 +367,7 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     if args.only == "adjoint":
-        print(json.dumps({"nonlinear_gradient": adjoint_bench(args.adjoint_steps)}), flush=True)
+        print(json.dumps({"nonlinear_gradient": adjoint_bench(args.adjoint_steps, args.adjoint_runs)}), flush=True)
         return 0
     if args.only == "batch":
         print(json.dumps({"batch": batch_bench(args.batch_steps)}), flush=True)
@@ -523,7 +524,7 @@ def main() -> int:
         out["batch"] = batch_bench(args.batch_steps)
     if not args.no_adjoint:
         torch.cuda.empty_cache()
-        out["nonlinear_gradient"] = adjoint_bench(args.adjoint_steps)
+        out["nonlinear_gradient"] = adjoint_bench(args.adjoint_steps, args.adjoint_runs)
     print(json.dumps(out), flush=True)
     return 0
 
