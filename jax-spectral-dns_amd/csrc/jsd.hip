@@ -1692,6 +1692,7 @@ struct jsd_plan {
   double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
   int adjoint;         // the discrete-adjoint tables are present
   double2* SS;         // adjoint of a nonlinear step: the three recomputed substep states
+  double2* TS;         // and the inverse x transforms of substeps 0 and 1, kept from the recomputation
   double *DEO, *DOE, *D2E, *D2O, *RBP, *LIP, *POP, *CW, *RW, *VAP, *VINV;  // parity blocks of the solve
   double *EVF, *EIF, *LAM;  // eigen-form solve: E, E^-1 fragments [even, odd], eigenvalues [2][PH]
   double *FF, *GF, *CWE, *RWE;  // E^-1 D and D E fragments, wall columns / rows in eigen coordinates
@@ -2472,6 +2473,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   p->YS1T = nullptr;
   p->yst_grid = 0;
   p->SS = nullptr;
+  p->TS = nullptr;
   const bool adj = (d->kind == JSD_KIND_PERTURBATION && d->linearise) ||
                    (d->adjoint && (d->kind == JSD_KIND_NS || d->kind == JSD_KIND_PERTURBATION));
   p->adjoint = adj ? 1 : 0;
@@ -3471,7 +3473,9 @@ static int forward_inverse_x(jsd_plan* p, const double2* U, hipStream_t st) {
 
 // UB += (dH/du at Us)^T Hbar, Hbar in p->H (overwritten); the transpose of the
 // nonlinear term of the NS / nonlinear perturbation step.
-static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStream_t st) {
+// Tpre: the inverse x transforms at Us kept from the forward recomputation
+// (NULL: transformed here).  The adjoint products overwrite them.
+static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, double2* Tpre, hipStream_t st) {
   const jsd_plan_desc& d = p->d;
   const size_t fs = p->fs;
   const int M = p->M, N = p->N, Z = p->Z, ZT = p->ZT;
@@ -3488,9 +3492,19 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStre
     if (!ok) JSD_FAIL(JSD_ERR_ARG, "x transform length %d has no adjoint kernel", d.ppx);
     HIPCHK(hipGetLastError());
   }
-  // forward fields at the substep's state: DU, T
-  int rc = forward_inverse_x(p, Us, st);
-  if (rc) return rc;
+  // forward fields at the substep's state: T (and DU), unless kept from the recomputation
+  double2* const T0 = p->T;
+  if (Tpre) {
+    p->T = Tpre;
+  } else {
+    int rc = forward_inverse_x(p, Us, st);
+    if (rc) return rc;
+  }
+  struct Restore {
+    jsd_plan* p;
+    double2* t;
+    ~Restore() { p->T = t; }
+  } restore{p, T0};
   // (ii) products adjoint, in place over T
   {
     const int prod = d.kind == JSD_KIND_NS ? PROD_NS : PROD_PERT;
@@ -3522,10 +3536,11 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, hipStre
   return launch_acc_T(p, p->DU, UB, 3, (long)(2 * fs), st);
 }
 
-static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s, bool has_carry, hipStream_t st) {
+static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, double2* Tpre, int s, bool has_carry,
+                              hipStream_t st) {
   int rc = solve_T_stage(p, UB, s, has_carry, st);
   if (rc) return rc;
-  rc = nl_adjoint_terms(p, UB, Us, st);
+  rc = nl_adjoint_terms(p, UB, Us, Tpre, st);
   if (rc) return rc;
   std::swap(p->HV, p->HVo);
   std::swap(p->HG, p->HGo);
@@ -3536,7 +3551,14 @@ static int adjoint_substep_nl(jsd_plan* p, double2* UB, const double2* Us, int s
 // the three substep states the nonlinear adjoint recomputes, kept by the plan
 static int recompute_alloc(jsd_plan* p) {
   if (p->SS) return JSD_OK;
-  return dalloc(p, (void**)&p->SS, 9 * p->fs * sizeof(double2));
+  int rc = dalloc(p, (void**)&p->SS, 9 * p->fs * sizeof(double2));
+  if (rc) return rc;
+  // inverse x transforms of substeps 0 and 1 (6 packed transforms each), when the
+  // compile-time kernels run (their layout is the one nl_adjoint_terms reads)
+  p->TS = nullptr;
+  if (p->generic_fft) return JSD_OK;
+  const size_t ts = (size_t)p->d.ppx * p->N * p->ZT;
+  return dalloc(p, (void**)&p->TS, 2 * 6 * ts * sizeof(double2));
 }
 
 extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, double dPdx, void* stream) {
@@ -3555,14 +3577,24 @@ extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, do
   double2* S2 = p->SS + 6 * p->fs;
   HIPCHK(hipMemcpyAsync(S0, state, sb, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(S1, state, sb, hipMemcpyDeviceToDevice, st));
+  // the recomputed substeps 0 and 1 leave their inverse x transforms (the
+  // forward fields the adjoint of their nonlinear term is linearised about)
+  // in TS instead of the plan's T
+  const size_t ts = (size_t)p->d.ppx * p->N * p->ZT;
+  double2* T0 = p->T;
+  double2* Tk[3] = {p->TS, p->TS ? p->TS + 6 * ts : nullptr, nullptr};
+  if (Tk[0]) p->T = Tk[0];
   int rc = substep(p, S1, nullptr, 0, dPdx, st);
+  p->T = T0;
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(S2, S1, sb, hipMemcpyDeviceToDevice, st));
+  if (Tk[1]) p->T = Tk[1];
   rc = substep(p, S2, nullptr, 1, dPdx, st);
+  p->T = T0;
   if (rc) return rc;
   const double2* Ss[3] = {S0, S1, S2};
   for (int s = 2; s >= 0; --s) {
-    rc = adjoint_substep_nl(p, (double2*)grad, Ss[s], s, s < 2, st);
+    rc = adjoint_substep_nl(p, (double2*)grad, Ss[s], Tk[s], s, s < 2, st);
     if (rc) return rc;
   }
   return JSD_OK;
