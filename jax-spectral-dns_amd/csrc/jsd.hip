@@ -3524,13 +3524,22 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, double2
     HIPCHK(hipGetLastError());
   }
   // (iii) forward x transforms, Herm and derivative factors: odd rows, then even
-  for (int parity = 1; parity >= 0; --parity) {
+  if (p->xreg) {
     const bool ok = visit_x(p->fpx, [&](auto c) {
-      hipLaunchKernelGGL(k_adj_x_fwd_t<decltype(c)>, dim3(nqb, N, 3), dim3(256), lds_x, st, p->T, (long)ts, parity, UB,
-                         p->DU, p->H, (long)fs, p->fpx.tw, p->kx, p->kz, M, N, Z, ZT, p->s_inv);
+      hipLaunchKernelGGL(k_adj_x_fwd_r<decltype(c)>, dim3(8 * ((3 * nqb * N + 7) / 8)), dim3(256), lds_x, st, p->T,
+                         (long)ts, UB, p->DU, (long)fs, p->fpx.tw, p->kx, p->kz, M, N, Z, ZT, nqb, p->s_inv);
     });
     if (!ok) JSD_FAIL(JSD_ERR_ARG, "x transform length %d has no adjoint kernel", d.ppx);
     HIPCHK(hipGetLastError());
+  } else {
+    for (int parity = 1; parity >= 0; --parity) {
+      const bool ok = visit_x(p->fpx, [&](auto c) {
+        hipLaunchKernelGGL(k_adj_x_fwd_t<decltype(c)>, dim3(nqb, N, 3), dim3(256), lds_x, st, p->T, (long)ts, parity,
+                           UB, p->DU, p->H, (long)fs, p->fpx.tw, p->kx, p->kz, M, N, Z, ZT, p->s_inv);
+      });
+      if (!ok) JSD_FAIL(JSD_ERR_ARG, "x transform length %d has no adjoint kernel", d.ppx);
+      HIPCHK(hipGetLastError());
+    }
   }
   // (iv) ubar_c += D^T Dubar_c
   return launch_acc_T(p, p->DU, UB, 3, (long)(2 * fs), st);

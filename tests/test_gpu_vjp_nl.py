@@ -103,3 +103,27 @@ def test_recompute_chunks_give_the_same_gradient():
     J3, g3 = gain_and_grad(u0, plan, 6, checkpoint_every=4)
     assert J1 == J2 == J3
     assert rel(g1, g2) < 1e-14 and rel(g1, g3) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(24, 65, 16), (48, 33, 8), (96, 17, 8)])
+def test_register_adjoint_x_pass_matches_lds_kernels(monkeypatch, shape):
+    """The merged register transpose of the inverse x pass (k_adj_x_fwd_r,
+    the default) against the two-launch LDS kernels (JSD_XPASS=0): the same
+    nonlinear-step adjoint to rounding, x lengths 40, 75 and 150."""
+    from jax_spectral_dns_amd import KIND_NS
+
+    g, _ = pair(shape, 2 * np.pi, np.pi)
+    rng = np.random.default_rng(9)
+    y = torch.from_numpy(rng.standard_normal((3, g.nxm, g.ny, g.nzm)) + 1j * rng.standard_normal((3, g.nxm, g.ny, g.nzm))).to("cuda")
+    u = perturbation_field(g, seed=25, energy=1e-2).to("cuda")
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("JSD_XPASS", mode)
+        plan = StepPlan(g, 3000.0, 0.005, KIND_NS, adjoint=True)
+        a = y.clone()
+        plan.vjp_state(u, a, -2.0 / 3000.0)
+        out.append(a)
+    monkeypatch.delenv("JSD_XPASS", raising=False)
+    err = rel(out[0], out[1])
+    print(shape, "register vs LDS adjoint x pass rel", err)
+    assert err < 1e-13
