@@ -978,10 +978,10 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
 // t and k2) takes the 16-point DFT over n1, giving the physical point
 // z = 16 k1 + k2.  Two LDS exchanges per transform instead of three stages,
 // no LDS zero fill.  The forward transform of the two packed product rows
-// runs the same two steps and stores only the kept bins.  A workgroup takes
-// LN neighbouring y lines of one x: with LN = 2 and 6 rows the butterfly
-// steps fill 3 of 4 waves and the forward steps one whole wave (one line
-// leaves half of a wave idle in both).
+// runs the same two steps, each 16-point DFT on four lanes, and stores only
+// the kept bins.  A workgroup takes LN neighbouring y lines of one x: with
+// LN = 2 and 6 rows the inverse steps fill 3 of 4 waves and the forward
+// transforms all 4 (one wave per packed row).
 #define Z16_LD 17  // padded row (in double2) of the [t][n1][k2] exchange layout
 
 template <int NT, int LN = 1>
@@ -1007,6 +1007,19 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
   const int l0 = blockIdx.y * LN;  // lines l0 .. l0+LN-1; those >= N load zeros and store nothing
   const int KZ = Z / 2;
   const int nneg = Z - KZ - 1;  // stored negative bins
+  // lanes of the forward transforms (below): their twiddles are loaded here,
+  // ahead of the inverse, so that no table load waits inside the forward steps
+  const bool fwd = threadIdx.x < 128 * LN;
+  const int fli = threadIdx.x >> 7, fo = (threadIdx.x >> 6) & 1;
+  const int fj = (threadIdx.x >> 4) & 3, fi = threadIdx.x & 15;
+  double2 fw16[4], fw256[4];  // w16^(fj k0), w256^(fi (fj + 4 k1))
+  if (fwd) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      fw16[k] = tw[16 * fj * k];
+      fw256[k] = tw[fi * (fj + 4 * k)];
+    }
+  }
   double2 v[16];
   // ---- inverse, step 1
   if (threadIdx.x < LN * NT * 16) {
@@ -1079,45 +1092,69 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
     }
   }
   __syncthreads();
-  // ---- forward, step 1 (rows o = 0, 1 of each line)
-  if (threadIdx.x < 32 * LN) {
-    const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, n1 = threadIdx.x & 15;
+  // ---- forward: the same 16 x 16 split, each 16-point DFT spread over four
+  // lanes as 4 x 4 (n = n0 + 4 n1, k = k0 + 4 k1: a 4-point DFT over n1, the
+  // twiddle w16^(n0 k0), an exchange inside the wave, a 4-point DFT over
+  // n0).  One wave per transform (line li, packed row o), so the two rows of
+  // LN lines run on 2 LN waves.  Per line: Y' [o][a][d] (16 x 17 padded) from
+  // 0, the quad exchange [o][4 x 4][16] from 2 * 272.
+  constexpr int FY = 16 * Z16_LD, FS = 2 * FY;
+  static_assert(128 * LN <= NTH && FS + 2 * 256 <= REG, "forward transform layout exceeds the workgroup");
+  double2* const frow = lds + fli * REG + fo * P;
+  double2* const fY = lds + fli * REG + fo * FY;
+  double2* const fS = lds + fli * REG + FS + fo * 256;
+  double2 u4[4];
+  // step 1, first half: lane (a = fi, n0 = fj) takes x[a + 16 (n0 + 4 n1)]
+  if (fwd) {
 #pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) v[n2] = lds[li * REG + o * P + n1 + 16 * n2];
-    dft16<-1>(v);
+    for (int n1 = 0; n1 < 4; ++n1) u4[n1] = frow[fi + 16 * fj + 64 * n1];
+    dft4(u4, -1.0);
+#pragma unroll
+    for (int k0 = 1; k0 < 4; ++k0) u4[k0] = cmul(u4[k0], fw16[k0]);
+#pragma unroll
+    for (int k0 = 0; k0 < 4; ++k0) fS[(fj * 4 + k0) * 16 + fi] = u4[k0];
   }
   __syncthreads();
-  if (threadIdx.x < 32 * LN) {
-    const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, n1 = threadIdx.x & 15;
-    {  // twiddles as running powers, as in the inverse
-      const double2 w1 = tw[n1], w8 = tw[8 * n1];
-      lds[li * REG + (o * 16 + n1) * Z16_LD] = v[0];
-      double2 wk = w1;
+  // step 1, second half: lane (a, k0 = fj) gives Y[a][d], d = k0 + 4 k1,
+  // times w256^(a d)
+  if (fwd) {
 #pragma unroll
-      for (int k2 = 1; k2 < 16; ++k2) {
-        if (k2 == 8) wk = w8;
-        lds[li * REG + (o * 16 + n1) * Z16_LD + k2] = cmul(v[k2], wk);
-        if (k2 != 7) wk = cmul(wk, w1);
-      }
+    for (int n0 = 0; n0 < 4; ++n0) u4[n0] = fS[(n0 * 4 + fj) * 16 + fi];
+    dft4(u4, -1.0);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      const int d = fj + 4 * k1;
+      fY[fi * Z16_LD + d] = fi * d == 0 ? u4[k1] : cmul(u4[k1], fw256[k1]);
     }
   }
   __syncthreads();
-  // ---- forward, step 2: kept bins to V
-  if (threadIdx.x < 32 * LN) {
-    const int li = threadIdx.x >> 5, o = (threadIdx.x >> 4) & 1, k2 = threadIdx.x & 15;
-    const int l = l0 + li;
+  // step 2, first half: lane (d = fi, a0 = fj) takes Y'[a0 + 4 a1][d]
+  if (fwd) {
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[li * REG + (o * 16 + n1) * Z16_LD + k2];
-    dft16<-1>(v);
+    for (int a1 = 0; a1 < 4; ++a1) u4[a1] = fY[(fj + 4 * a1) * Z16_LD + fi];
+    dft4(u4, -1.0);
+#pragma unroll
+    for (int c0 = 1; c0 < 4; ++c0) u4[c0] = cmul(u4[c0], fw16[c0]);
+#pragma unroll
+    for (int c0 = 0; c0 < 4; ++c0) fS[(fj * 4 + c0) * 16 + fi] = u4[c0];
+  }
+  __syncthreads();
+  // step 2, second half: lane (d, c0 = fj) gives bins 16 (c0 + 4 c1) + d;
+  // the kept bins are stored (64 consecutive lanes, consecutive bins)
+  if (fwd) {
+#pragma unroll
+    for (int a0 = 0; a0 < 4; ++a0) u4[a0] = fS[(a0 * 4 + fj) * 16 + fi];
+    dft4(u4, -1.0);
+    const int l = l0 + fli;
     if (l < N) {
       const size_t line = ((size_t)x * N + l) * ZT;
 #pragma unroll
-      for (int k1 = 0; k1 < 16; ++k1) {
-        const int b = 16 * k1 + k2;
+      for (int c1 = 0; c1 < 4; ++c1) {
+        const int b = 16 * fj + 64 * c1 + fi;
         int j = -1;
         if (b <= KZ) j = b;
         else if (b >= P - nneg) j = b - P + Z;
-        if (j >= 0) st_nt(&V[o * vstride + line + j], v[k1]);
+        if (j >= 0) st_nt(&V[fo * vstride + line + j], u4[c1]);
       }
     }
   }
