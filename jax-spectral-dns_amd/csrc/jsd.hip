@@ -971,17 +971,17 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
   }
 }
 
-// z pass for P = 256 = 16 x 16 with the butterflies in registers.  With
-// n = n1 + 16 n2 and k = 16 k1 + k2: step 1 (one thread per row t and n1)
-// reads x[n1 + 16 n2] straight from HBM (zero padding by index), takes the
-// 16-point DFT over n2 and multiplies by w256^(n1 k2); step 2 (one thread per
-// t and k2) takes the 16-point DFT over n1, giving the physical point
-// z = 16 k1 + k2.  Two LDS exchanges per transform instead of three stages,
-// no LDS zero fill.  The forward transform of the two packed product rows
-// runs the same two steps, each 16-point DFT on four lanes, and stores only
-// the kept bins.  A workgroup takes LN neighbouring y lines of one x: with
-// LN = 2 and 6 rows the inverse steps fill 3 of 4 waves and the forward
-// transforms all 4 (one wave per packed row).
+// z pass for P = 16 N1 (N1 = 16: P = 256; N1 = 8: P = 128) with the
+// butterflies in registers.  With n = n1 + N1 n2 and k = 16 k1 + k2: step 1
+// (one thread per row t and n1 < N1) reads x[n1 + N1 n2] straight from HBM
+// (zero padding by index), takes the 16-point DFT over n2 and multiplies by
+// wP^(n1 k2); step 2 (one thread per t and k2 < 16) takes the N1-point DFT
+// over n1, giving the physical point z = 16 k1 + k2.  Two LDS exchanges per
+// transform instead of three or four stages, no LDS zero fill.  The forward
+// transform of the two packed product rows takes the same split, each DFT
+// spread over lanes (below), and stores only the kept bins.  A workgroup
+// takes LN neighbouring y lines of one x: with LN = 2 and 6 rows the 16-lane
+// steps fill 3 of 4 waves.
 #define Z16_LD 17  // padded row (in double2) of the [t][n1][k2] exchange layout
 
 template <int NT, int LN = 1>
@@ -989,47 +989,56 @@ __host__ __device__ constexpr int z256_threads() {
   return NT * 16 * LN > 128 ? (LN == 1 ? NT * 16 : 256) : 128;
 }
 
-// PZ: at most 129 stored kz (|kz| <= 64), so step 1 of the inverse sees
-// zeros at n2 = 5..11 (dft16_pz).
-template <int NT, int PROD, bool PZ, int LN = 1>
-__global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
+// PZ (P = 256 only): at most 129 stored kz (|kz| <= 64), so step 1 of the
+// inverse sees zeros at n2 = 5..11 (dft16_pz).
+template <int P, int NT, int PROD, bool PZ, int LN = 1>
+__global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void k_fft_z_mid_r(
     const double2* __restrict__ T, long tstride, double2* __restrict__ V, long vstride,
     const double2* __restrict__ tw, int N, int Z, int ZT, double clin, const double* __restrict__ Ub,
     const double* __restrict__ dUb) {
-  constexpr int P = 256;
+  static_assert(P == 128 || P == 256, "register z pass: P = 128 or 256");
+  static_assert(!PZ || P == 256, "the pruned first stage assumes P = 256");
+  constexpr int N1 = P / 16;
   constexpr int NTH = z256_threads<NT, LN>();
-  static_assert(NT * 16 * LN <= NTH && 32 * LN <= NTH, "rows of the workgroup exceed its threads");
+  static_assert(NT * 16 * LN <= NTH, "rows of the workgroup exceed its threads");
   constexpr int ITEMS = (LN * P + NTH - 1) / NTH;
   // per line: [t][n1][k2] (padded) for the exchanges, [t][z] for the products
-  constexpr int REG = NT * 16 * Z16_LD > P * NT ? NT * 16 * Z16_LD : P * NT;
+  constexpr int REG = NT * N1 * Z16_LD > P * NT ? NT * N1 * Z16_LD : P * NT;
   __shared__ double2 lds[LN * REG];
   const int x = blockIdx.x;
   const int l0 = blockIdx.y * LN;  // lines l0 .. l0+LN-1; those >= N load zeros and store nothing
   const int KZ = Z / 2;
   const int nneg = Z - KZ - 1;  // stored negative bins
-  // lanes of the forward transforms (below): their twiddles are loaded here,
-  // ahead of the inverse, so that no table load waits inside the forward steps
-  const bool fwd = threadIdx.x < 128 * LN;
-  const int fli = threadIdx.x >> 7, fo = (threadIdx.x >> 6) & 1;
-  const int fj = (threadIdx.x >> 4) & 3, fi = threadIdx.x & 15;
-  double2 fw16[4], fw256[4];  // w16^(fj k0), w256^(fi (fj + 4 k1))
+  // lanes of the forward transforms (below): FL per packed row, line fli,
+  // row fo, position fi in the row's DFT, quad / pair position fj; their
+  // twiddles are loaded here, ahead of the inverse, so that no table load
+  // waits inside the forward steps
+  constexpr int FL = P / 4;
+  static_assert(2 * FL * LN <= NTH, "forward lanes exceed the workgroup");
+  const bool fwd = (int)threadIdx.x < 2 * FL * LN;
+  const int fli = threadIdx.x / (2 * FL), fo = (threadIdx.x / FL) & 1;
+  const int fr = threadIdx.x % FL;
+  const int fj = fr / N1, fi = fr % N1;  // step 1: a = fi < N1, n0 = fj < 4
+  const int f2d = fr & 15, f2j = fr >> 4;  // step 2: d = f2d < 16, f2j < FL / 16
+  double2 fw16[4], fw1[4], fw2[4];  // w16^(fj k0); wP^(fi (fj + 4 k1)); step 2 twiddles
   if (fwd) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      fw16[k] = tw[16 * fj * k];
-      fw256[k] = tw[fi * (fj + 4 * k)];
+      fw16[k] = tw[N1 * fj * k];
+      fw1[k] = tw[fi * (fj + 4 * k)];
+      fw2[k] = tw[16 * f2j * k];  // w16^(a0 c0) (P = 256) or w8^(a0 c0) (P = 128)
     }
   }
   double2 v[16];
   // ---- inverse, step 1
-  if (threadIdx.x < LN * NT * 16) {
-    const int li = threadIdx.x / (NT * 16), r = threadIdx.x % (NT * 16);
-    const int t = r >> 4, n1 = r & 15;
+  if (threadIdx.x < LN * NT * N1) {
+    const int li = threadIdx.x / (NT * N1), r = threadIdx.x % (NT * N1);
+    const int t = r / N1, n1 = r % N1;
     const int l = l0 + li;
     const size_t line = ((size_t)x * N + (l < N ? l : 0)) * ZT;
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
-      const int n = n1 + 16 * n2;
+      const int n = n1 + N1 * n2;
       int j = -1;
       if (n <= KZ) j = n;
       else if (n >= P - nneg) j = n - P + Z;
@@ -1037,17 +1046,17 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
     }
     if constexpr (PZ) dft16_pz<+1>(v);
     else dft16<+1>(v);
-    // twiddles w256^(n1 k2) = w^k2 (w = w256^n1 from the table) as running
+    // twiddles wP^(n1 k2) = w^k2 (w = wP^n1 from the table) as running
     // powers, restarted from the table at k2 = 8: two loads and two live
     // twiddles instead of fifteen (the powers agree with the table to ~8 ulp)
     {
       const double2 w1 = cconj(tw[n1]), w8 = cconj(tw[8 * n1]);
-      lds[li * REG + (t * 16 + n1) * Z16_LD] = v[0];
+      lds[li * REG + (t * N1 + n1) * Z16_LD] = v[0];
       double2 wk = w1;
 #pragma unroll
       for (int k2 = 1; k2 < 16; ++k2) {
         if (k2 == 8) wk = w8;
-        lds[li * REG + (t * 16 + n1) * Z16_LD + k2] = cmul(v[k2], wk);
+        lds[li * REG + (t * N1 + n1) * Z16_LD + k2] = cmul(v[k2], wk);
         if (k2 != 7) wk = cmul(wk, w1);
       }
     }
@@ -1058,15 +1067,16 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
     const int li = threadIdx.x / (NT * 16), r = threadIdx.x % (NT * 16);
     const int t = r >> 4, k2 = r & 15;
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) v[n1] = lds[li * REG + (t * 16 + n1) * Z16_LD + k2];
-    dft16<+1>(v);
+    for (int n1 = 0; n1 < N1; ++n1) v[n1] = lds[li * REG + (t * N1 + n1) * Z16_LD + k2];
+    if constexpr (N1 == 16) dft16<+1>(v);
+    else dft8(v, 1.0);
   }
   __syncthreads();
   if (threadIdx.x < LN * NT * 16) {
     const int li = threadIdx.x / (NT * 16), r = threadIdx.x % (NT * 16);
     const int t = r >> 4, k2 = r & 15;
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) lds[li * REG + t * P + 16 * k1 + k2] = v[k1];
+    for (int k1 = 0; k1 < N1; ++k1) lds[li * REG + t * P + 16 * k1 + k2] = v[k1];
   }
   __syncthreads();
   // ---- products at each physical point
@@ -1092,69 +1102,103 @@ __global__ __launch_bounds__((z256_threads<NT, LN>())) void k_fft_z_mid_256(
     }
   }
   __syncthreads();
-  // ---- forward: the same 16 x 16 split, each 16-point DFT spread over four
-  // lanes as 4 x 4 (n = n0 + 4 n1, k = k0 + 4 k1: a 4-point DFT over n1, the
-  // twiddle w16^(n0 k0), an exchange inside the wave, a 4-point DFT over
-  // n0).  One wave per transform (line li, packed row o), so the two rows of
-  // LN lines run on 2 LN waves.  Per line: Y' [o][a][d] (16 x 17 padded) from
-  // 0, the quad exchange [o][4 x 4][16] from 2 * 272.
-  constexpr int FY = 16 * Z16_LD, FS = 2 * FY;
-  static_assert(128 * LN <= NTH && FS + 2 * 256 <= REG, "forward transform layout exceeds the workgroup");
+  // ---- forward, n = a + N1 b, k = 16 c + d:
+  //   X[16 c + d] = sum_a wP^(a d) wN1^(a c) sum_b x[a + N1 b] w16^(b d).
+  // Step 1, per a: the 16-point DFT over b on four lanes as 4 x 4
+  // (b = n0 + 4 n1, d = k0 + 4 k1: a 4-point DFT over n1, the twiddle
+  // w16^(n0 k0), an exchange inside the wave, a 4-point DFT over n0), then
+  // wP^(a d).  Step 2, per d: the N1-point DFT over a, on four lanes as
+  // 4 x 4 (N1 = 16) or on two lanes as 4 x 2 (N1 = 8).  One wave (P = 256)
+  // or half a wave (P = 128) per transform (line fli, packed row fo).  Per
+  // line: Y' [o][a][d] (N1 x 17 padded) from 0, the lane exchange [o][P]
+  // from 2 N1 17.
+  constexpr int FY = N1 * Z16_LD, FS = 2 * FY;
+  static_assert(FS + 2 * P <= REG, "forward transform layout exceeds the workgroup");
   double2* const frow = lds + fli * REG + fo * P;
   double2* const fY = lds + fli * REG + fo * FY;
-  double2* const fS = lds + fli * REG + FS + fo * 256;
+  double2* const fS = lds + fli * REG + FS + fo * P;
   double2 u4[4];
-  // step 1, first half: lane (a = fi, n0 = fj) takes x[a + 16 (n0 + 4 n1)]
+  // step 1, first half: lane (a = fi, n0 = fj) takes x[a + N1 (n0 + 4 n1)]
   if (fwd) {
 #pragma unroll
-    for (int n1 = 0; n1 < 4; ++n1) u4[n1] = frow[fi + 16 * fj + 64 * n1];
+    for (int n1 = 0; n1 < 4; ++n1) u4[n1] = frow[fi + N1 * fj + 4 * N1 * n1];
     dft4(u4, -1.0);
 #pragma unroll
     for (int k0 = 1; k0 < 4; ++k0) u4[k0] = cmul(u4[k0], fw16[k0]);
 #pragma unroll
-    for (int k0 = 0; k0 < 4; ++k0) fS[(fj * 4 + k0) * 16 + fi] = u4[k0];
+    for (int k0 = 0; k0 < 4; ++k0) fS[(fj * 4 + k0) * N1 + fi] = u4[k0];
   }
   __syncthreads();
   // step 1, second half: lane (a, k0 = fj) gives Y[a][d], d = k0 + 4 k1,
-  // times w256^(a d)
+  // times wP^(a d)
   if (fwd) {
 #pragma unroll
-    for (int n0 = 0; n0 < 4; ++n0) u4[n0] = fS[(n0 * 4 + fj) * 16 + fi];
+    for (int n0 = 0; n0 < 4; ++n0) u4[n0] = fS[(n0 * 4 + fj) * N1 + fi];
     dft4(u4, -1.0);
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) {
       const int d = fj + 4 * k1;
-      fY[fi * Z16_LD + d] = fi * d == 0 ? u4[k1] : cmul(u4[k1], fw256[k1]);
+      fY[fi * Z16_LD + d] = fi * d == 0 ? u4[k1] : cmul(u4[k1], fw1[k1]);
     }
   }
   __syncthreads();
-  // step 2, first half: lane (d = fi, a0 = fj) takes Y'[a0 + 4 a1][d]
-  if (fwd) {
+  if constexpr (N1 == 16) {
+    // step 2, first half: lane (d, a0 = f2j) takes Y'[a0 + 4 a1][d]
+    if (fwd) {
 #pragma unroll
-    for (int a1 = 0; a1 < 4; ++a1) u4[a1] = fY[(fj + 4 * a1) * Z16_LD + fi];
-    dft4(u4, -1.0);
+      for (int a1 = 0; a1 < 4; ++a1) u4[a1] = fY[(f2j + 4 * a1) * Z16_LD + f2d];
+      dft4(u4, -1.0);
 #pragma unroll
-    for (int c0 = 1; c0 < 4; ++c0) u4[c0] = cmul(u4[c0], fw16[c0]);
+      for (int c0 = 1; c0 < 4; ++c0) u4[c0] = cmul(u4[c0], fw2[c0]);
 #pragma unroll
-    for (int c0 = 0; c0 < 4; ++c0) fS[(fj * 4 + c0) * 16 + fi] = u4[c0];
+      for (int c0 = 0; c0 < 4; ++c0) fS[(f2j * 4 + c0) * 16 + f2d] = u4[c0];
+    }
+    __syncthreads();
+    // step 2, second half: lane (d, c0 = f2j) gives bins 16 (c0 + 4 c1) + d
+    if (fwd) {
+#pragma unroll
+      for (int a0 = 0; a0 < 4; ++a0) u4[a0] = fS[(a0 * 4 + f2j) * 16 + f2d];
+      dft4(u4, -1.0);
+    }
+  } else {
+    // step 2, first half: lane (d, a0 = f2j < 2) takes Y'[a0 + 2 a1][d],
+    // a 4-point DFT over a1 gives c0, times w8^(a0 c0)
+    if (fwd) {
+#pragma unroll
+      for (int a1 = 0; a1 < 4; ++a1) u4[a1] = fY[(f2j + 2 * a1) * Z16_LD + f2d];
+      dft4(u4, -1.0);
+#pragma unroll
+      for (int c0 = 1; c0 < 4; ++c0) u4[c0] = cmul(u4[c0], fw2[c0]);
+#pragma unroll
+      for (int c0 = 0; c0 < 4; ++c0) fS[(f2j * 4 + c0) * 16 + f2d] = u4[c0];
+    }
+    __syncthreads();
+    // step 2, second half: lane (d, e = f2j) takes c0 = 2 e, 2 e + 1 of both
+    // a0 and a 2-point DFT over a0 gives c1: u4[2 m + c1] holds bin
+    // 16 (2 e + m + 4 c1) + d
+    if (fwd) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        double2 t2[2] = {fS[(0 * 4 + 2 * f2j + m) * 16 + f2d], fS[(1 * 4 + 2 * f2j + m) * 16 + f2d]};
+        dft2(t2);
+        u4[2 * m] = t2[0];
+        u4[2 * m + 1] = t2[1];
+      }
+    }
   }
-  __syncthreads();
-  // step 2, second half: lane (d, c0 = fj) gives bins 16 (c0 + 4 c1) + d;
-  // the kept bins are stored (64 consecutive lanes, consecutive bins)
   if (fwd) {
-#pragma unroll
-    for (int a0 = 0; a0 < 4; ++a0) u4[a0] = fS[(a0 * 4 + fj) * 16 + fi];
-    dft4(u4, -1.0);
     const int l = l0 + fli;
     if (l < N) {
       const size_t line = ((size_t)x * N + l) * ZT;
 #pragma unroll
-      for (int c1 = 0; c1 < 4; ++c1) {
-        const int b = 16 * fj + 64 * c1 + fi;
+      for (int q = 0; q < 4; ++q) {
+        // P = 256: c = f2j + 4 q; P = 128: c = 2 f2j + (q >> 1) + 4 (q & 1)
+        const int c = N1 == 16 ? f2j + 4 * q : 2 * f2j + (q >> 1) + 4 * (q & 1);
+        const int b = 16 * c + f2d;
         int j = -1;
         if (b <= KZ) j = b;
         else if (b >= P - nneg) j = b - P + Z;
-        if (j >= 0) st_nt(&V[fo * vstride + line + j], u4[c1]);
+        if (j >= 0) st_nt(&V[fo * vstride + line + j], u4[q]);
       }
     }
   }
@@ -2807,26 +2851,30 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   HIPCHK(hipGetLastError());
   prof_begin(p, 2, st);
   const bool z_256 = ct && plan_is<RZ256>(p->fpz);
-  if (z_256) {
+  const bool z_128 = ct && plan_is<RZ128>(p->fpz);
+  if (z_256 || z_128) {
     const dim3 grid(d.ppx, N), grid2(d.ppx, (N + 1) / 2);
-    auto launch = [&](auto pz) {
+    auto launch = [&](auto pc, auto pz) {
+      constexpr int P = decltype(pc)::value;
       constexpr bool PZ = decltype(pz)::value;
       if (prod == PROD_NS)
-        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_NS, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_NS, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
                            (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
       else if (prod == PROD_PERT)
-        hipLaunchKernelGGL((k_fft_z_mid_256<6, PROD_PERT, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_PERT, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
                            (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
       else
-        hipLaunchKernelGGL((k_fft_z_mid_256<12, PROD_DUAL, PZ>), grid, dim3(z256_threads<12>()), 0, st, p->T,
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 12, PROD_DUAL, PZ>), grid, dim3(z256_threads<12>()), 0, st, p->T,
                            (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
     };
-    if (Z <= 129)
-      launch(std::true_type());
+    if (z_128)
+      launch(std::integral_constant<int, 128>(), std::false_type());
+    else if (Z <= 129)
+      launch(std::integral_constant<int, 256>(), std::true_type());
     else
-      launch(std::false_type());
+      launch(std::integral_constant<int, 256>(), std::false_type());
   }
-  const bool z_ct = z_256 || (ct && visit_z(p->fpz, [&](auto c) {
+  const bool z_ct = z_256 || z_128 || (ct && visit_z(p->fpz, [&](auto c) {
     using C = decltype(c);
     const size_t lds = (size_t)C::P * ntr * sizeof(double2);
     const dim3 grid(d.ppx, N), block(zmid_threads(C::P));

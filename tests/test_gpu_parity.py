@@ -70,12 +70,13 @@ def test_nonlinear_ns(shape, lx, lz):
     assert rel(conv, rc) < 1e-12
 
 
-@pytest.mark.parametrize("shape", [(16, 33, 160), (16, 33, 96), (8, 33, 128)])
+@pytest.mark.parametrize("shape", [(16, 33, 160), (16, 33, 96), (8, 33, 128), (16, 33, 80), (8, 32, 48), (16, 33, 64)])
 def test_nonlinear_ns_z256(shape):
-    """z transform length 256 (the 16 x 16 register kernel) against the oracle;
-    97 and 129 stored kz take its pruned first stage (inputs n2 = 5..11 zero)."""
+    """z transform lengths 256 (16 x 16) and 128 (8 x 16) of the register
+    kernel against the oracle; at length 256, 97 and 129 stored kz take its
+    pruned first stage (inputs n2 = 5..11 zero)."""
     g, d = pair(shape)
-    assert g.ppz == 256
+    assert g.ppz == (256 if shape[2] >= 96 else 128)
     plan = StepPlan(g, 5000.0, 0.005, KIND_NS)
     u = ns_field(g, seed=12)
     hv, hg, conv = plan.nonlinear(u.to(DEV))
@@ -139,13 +140,13 @@ def test_nonlinear_perturbation(shape, lx, lz, linearise):
 
 
 @pytest.mark.parametrize("coupling_term", [True, False])
-@pytest.mark.parametrize("shape", [(16, 33, 96), (16, 33, 160), (16, 32, 96)])
+@pytest.mark.parametrize("shape", [(16, 33, 96), (16, 33, 160), (16, 32, 96), (16, 33, 80), (16, 32, 64)])
 def test_nonlinear_perturbation_z256(shape, coupling_term):
-    """The perturbation product on the z transform of length 256 (two y lines
-    per workgroup, base flow looked up per line, the second line masked when
-    Ny is odd); 161 stored kz take the unpruned first stage."""
+    """The perturbation product on the register z transform, lengths 256 and
+    128 (two y lines per workgroup, base flow looked up per line, the second
+    line masked when Ny is odd); 161 stored kz take the unpruned first stage."""
     g, d = pair(shape)
-    assert g.ppz == 256
+    assert g.ppz == (256 if shape[2] >= 96 else 128)
     plan = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION, linearise=False, coupling_term=coupling_term)
     u = perturbation_field(g, seed=14, energy=1e-2)
     hv, hg, conv = plan.nonlinear(u.to(DEV))
@@ -156,11 +157,12 @@ def test_nonlinear_perturbation_z256(shape, coupling_term):
     assert rel(conv, rc) < 1e-12
 
 
-@pytest.mark.parametrize("shape", [(16, 33, 96), (16, 33, 160)])
+@pytest.mark.parametrize("shape", [(16, 33, 96), (16, 33, 160), (16, 33, 80), (8, 32, 48)])
 def test_nonlinear_dual_z256(shape):
-    """The dual product (12 packed transforms) on the z transform of length 256."""
+    """The dual product (12 packed transforms) on the register z transform,
+    lengths 256 and 128."""
     g, d = pair(shape)
-    assert g.ppz == 256
+    assert g.ppz == (256 if shape[2] >= 96 else 128)
     fwd = StepPlan(g, 3000.0, 0.01, KIND_PERTURBATION)
     plan = StepPlan(g, 3000.0, 0.01, KIND_DUAL, ops=fwd.ops, explicit_dt=-0.01, plan_re=-3000.0)
     v = perturbation_field(g, seed=15, energy=1e-2)
