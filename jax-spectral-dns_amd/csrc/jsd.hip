@@ -1770,6 +1770,8 @@ struct jsd_plan {
   double* YSZ;        // zeros read by its lanes outside the data
   YsArgs ys1, ys2;
   int ys_grid[2];     // persistent grid of k_ypar_stream for D, D^2 (0: k_ypar instead)
+  int zreg;           // 1: register z passes for lengths 128 and 256 (default); 0: the LDS kernels
+                      // (JSD_ZPASS=lds, a diagnostic switch)
   double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
   int adjoint;         // the discrete-adjoint tables are present
   double2* SS;         // adjoint of a nonlinear step: the three recomputed substep states
@@ -2383,6 +2385,8 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     if (d->ny > 130) p->solve8 = std::min(p->solve8, 1);  // one interior row pair per thread
     const char* envx = getenv("JSD_XPASS");
     p->xreg = (envx && envx[0] >= '0' && envx[0] <= '2') ? envx[0] - '0' : 1;
+    const char* envzp = getenv("JSD_ZPASS");
+    p->zreg = (envzp && strcmp(envzp, "lds") == 0) ? 0 : 1;
   }
   p->M = M;
   p->N = N;
@@ -2850,8 +2854,8 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
   prof_end(p, 1, st);
   HIPCHK(hipGetLastError());
   prof_begin(p, 2, st);
-  const bool z_256 = ct && plan_is<RZ256>(p->fpz);
-  const bool z_128 = ct && plan_is<RZ128>(p->fpz);
+  const bool z_256 = ct && p->zreg && plan_is<RZ256>(p->fpz);
+  const bool z_128 = ct && p->zreg && plan_is<RZ128>(p->fpz);
   if (z_256 || z_128) {
     const dim3 grid(d.ppx, N), grid2(d.ppx, (N + 1) / 2);
     auto launch = [&](auto pc, auto pz) {
@@ -3595,7 +3599,28 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, double2
     const int prod = d.kind == JSD_KIND_NS ? PROD_NS : PROD_PERT;
     const double clin = 1.0;
     const dim3 grid(d.ppx, N);
-    const bool ok = visit_z(p->fpz, [&](auto c) {
+    const bool zr = !p->generic_fft && p->zreg && (plan_is<RZ128>(p->fpz) || plan_is<RZ256>(p->fpz));
+    if (zr) {
+      // register form, two y lines per workgroup
+      const dim3 grid2(d.ppx, (N + 1) / 2);
+      auto launch = [&](auto pc, auto pz) {
+        constexpr int P = decltype(pc)::value;
+        constexpr bool PZ = decltype(pz)::value;
+        if (prod == PROD_NS)
+          hipLaunchKernelGGL((k_adj_z_r<P, PROD_NS, PZ, 2>), grid2, dim3(z256_threads<8, 2>()), 0, st, p->T,
+                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+        else
+          hipLaunchKernelGGL((k_adj_z_r<P, PROD_PERT, PZ, 2>), grid2, dim3(z256_threads<8, 2>()), 0, st, p->T,
+                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+      };
+      if (plan_is<RZ128>(p->fpz))
+        launch(std::integral_constant<int, 128>(), std::false_type());
+      else if (Z <= 129)
+        launch(std::integral_constant<int, 256>(), std::true_type());
+      else
+        launch(std::integral_constant<int, 256>(), std::false_type());
+    }
+    const bool ok = zr || visit_z(p->fpz, [&](auto c) {
       using C = decltype(c);
       const size_t lds = (size_t)C::P * 8 * sizeof(double2);
       if (prod == PROD_NS)

@@ -36,8 +36,10 @@ def _oracle_grad(sp, d, u0, n):
     return float(Jr), torch.view_as_complex(x.grad)
 
 
-@pytest.mark.parametrize("shape,lz", [((24, 33, 16), np.pi), ((24, 65, 16), np.pi)])
+@pytest.mark.parametrize("shape,lz", [((24, 33, 16), np.pi), ((24, 65, 16), np.pi), ((24, 33, 48), np.pi)])
 def test_nonlinear_perturbation_gradient_against_oracle_autograd(shape, lz):
+    """(24, 33, 48): z transform length 128, the register z passes of the
+    forward and the adjoint."""
     g, d = pair(shape, 2 * np.pi, lz)
     re, dt, n = 3000.0, 0.005, 4
     u0 = perturbation_field(g, seed=0, energy=1e-2)
@@ -127,3 +129,34 @@ def test_register_adjoint_x_pass_matches_lds_kernels(monkeypatch, shape):
     err = rel(out[0], out[1])
     print(shape, "register vs LDS adjoint x pass rel", err)
     assert err < 1e-13
+
+
+@pytest.mark.parametrize("kind", [KIND_NS, KIND_PERTURBATION])
+@pytest.mark.parametrize("shape", [(24, 33, 64), (24, 32, 96)])
+def test_register_z_passes_match_lds_kernels(monkeypatch, shape, kind):
+    """The register z passes (k_fft_z_mid_r, k_adj_z_r; z lengths 128 and
+    256, the default) against the LDS kernels (JSD_ZPASS=lds): the same
+    nonlinear step and the same step adjoint to rounding."""
+    g, _ = pair(shape, 2 * np.pi, np.pi)
+    assert g.ppz == (256 if shape[2] >= 96 else 128)
+    rng = np.random.default_rng(10)
+    y = torch.from_numpy(rng.standard_normal((3, g.nxm, g.ny, g.nzm)) + 1j * rng.standard_normal((3, g.nxm, g.ny, g.nzm))).to(DEV)
+    u = perturbation_field(g, seed=26, energy=1e-2).to(DEV)
+    dpdx = -2.0 / 3000.0 if kind == KIND_NS else 0.0
+    steps, adjs = [], []
+    for mode in ("reg", "lds"):
+        if mode == "lds":
+            monkeypatch.setenv("JSD_ZPASS", "lds")
+        kw = {} if kind == KIND_NS else {"linearise": False}
+        plan = StepPlan(g, 3000.0, 0.005, kind, adjoint=True, **kw)
+        s = u.clone()
+        plan.step(s, dpdx, 2)
+        steps.append(s)
+        a = y.clone()
+        plan.vjp_state(u, a, dpdx)
+        adjs.append(a)
+    monkeypatch.delenv("JSD_ZPASS", raising=False)
+    es, ea = rel(steps[0], steps[1]), rel(adjs[0], adjs[1])
+    print(shape, kind, "register vs LDS z passes: step rel %.3e, adjoint rel %.3e" % (es, ea))
+    assert es < 1e-13
+    assert ea < 1e-13
