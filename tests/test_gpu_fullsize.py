@@ -82,6 +82,34 @@ def test_cfg1_ten_steps_against_oracle():
         assert err < 1e-11 * (n + 1)
 
 
+def test_cfg1_200_steps_against_oracle_energy_history():
+    """The full cfg1 horizon: 200 steps of the GPU step against the oracle's
+    energy after every step and its (0,0) mean-flow lines of u and w after
+    1, 10, 100 and 200 steps (tests/golden/make_golden.py cfg1)."""
+    path = os.path.join(HERE, "golden", "cfg1_ns_200_energy.npz")
+    fx = np.load(path)
+    re, dt, n = fx["params"][:3]
+    g, u0 = _cfg1()
+    plan = StepPlan(g, re, dt, KIND_NS)
+    u = u0.to(DEV)
+    e0 = plan.energy(u)
+    assert abs(e0 - fx["energy"][0]) <= 1e-13 * fx["energy"][0]
+    worst = 0.0
+    for t in range(int(n)):
+        plan.step(u, -2.0 / re, 1)
+        e = plan.energy(u)
+        err = abs(e - fx["energy"][t + 1]) / fx["energy"][t + 1]
+        worst = max(worst, err)
+        assert err < 1e-11, (t + 1, err)
+        if t + 1 in (1, 10, 100, 200):
+            ref = torch.from_numpy(fx["mean%d" % (t + 1)])
+            mean = torch.stack([u[0, 0, :, 0], u[2, 0, :, 0]]).cpu()
+            merr = rel(mean, ref)
+            print("cfg1 step %d: energy rel %.2e, mean flow rel %.2e" % (t + 1, err, merr))
+            assert merr < 1e-11
+    print("cfg1 worst energy rel over 200 steps %.2e" % worst)
+
+
 def test_cfg1_properties_over_steps():
     g, u0 = _cfg1()
     re, dt = 5000.0, 0.005
