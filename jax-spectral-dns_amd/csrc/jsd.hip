@@ -538,6 +538,13 @@ __device__ __forceinline__ int slot_col(int c, int q0, int Z, int KZ, bool* ok) 
   *ok = (q <= KZ) && (!neg || q >= 1);
   return neg ? Z - q : q;
 }
+// Column of stored kz index j in the rows of the transformed fields (T, V,
+// the adjoint rows): negative kz sit ZCOL_SHIFT columns further right (the
+// row pitch ZT >= Z + ZCOL_SHIFT), so that with Z = 1 (mod 4) a block of QB
+// negative kz starts on a 64-byte boundary, as a block of positive kz does,
+// and the x passes write and read whole 64-byte segments.
+#define ZCOL_SHIFT 2
+__device__ __forceinline__ int zcol(int j, int KZ) { return j > KZ ? j + ZCOL_SHIFT : j; }
 __device__ __forceinline__ int slot_partner(int c, int q0) {
   const int q = q0 + (c % QB);
   return q == 0 ? c : (c ^ QB);
@@ -621,7 +628,7 @@ __global__ __launch_bounds__(256) void k_fft_x_inv(XInvSet set, int ntr, int nqb
     const int x = w / NSLOT;
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
-    if (ok) T[((size_t)x * N + l) * ZT + j] = buf[x * NSLOT + c];
+    if (ok) T[((size_t)x * N + l) * ZT + zcol(j, KZ)] = buf[x * NSLOT + c];
   }
 }
 
@@ -721,7 +728,7 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
     const int yb = r / nt, t = r - yb * nt;
     if (yb >= nyb) continue;
     const int n = (j <= KZ) ? j : P - (Z - j);
-    lds[PZ.iperm[n] * R + r] = T[t * tstride + ((size_t)x * N + l0 + yb) * ZT + j];
+    lds[PZ.iperm[n] * R + r] = T[t * tstride + ((size_t)x * N + l0 + yb) * ZT + zcol(j, KZ)];
   }
   __syncthreads();
   fft_lds(lds, R, R, 1, PZ, +1.0);
@@ -758,7 +765,7 @@ __global__ __launch_bounds__(256) void k_fft_z_mid(const double2* __restrict__ T
     const int yb = r / 2, o = r % 2;
     if (yb >= nyb) continue;
     const int n = (j <= KZ) ? j : P - (Z - j);
-    V[o * vstride + ((size_t)x * N + l0 + yb) * ZT + j] = lds[n * R2 + r];
+    V[o * vstride + ((size_t)x * N + l0 + yb) * ZT + zcol(j, KZ)] = lds[n * R2 + r];
   }
 }
 
@@ -784,7 +791,7 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd(const double2* __restrict__ V
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
     const int xi = PX.perm[pos];
-    buf[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * ZT + j] : make_double2(0.0, 0.0);
+    buf[pos * NSLOT + c] = ok ? Vt[((size_t)xi * N + l) * ZT + zcol(j, KZ)] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   fft_lds(buf, NSLOT, NSLOT, 1, PX, -1.0);
@@ -909,7 +916,7 @@ __global__ __launch_bounds__(256) void k_fft_x_inv_t(XInvSet set, int ntr, int n
     const int x = w / NSLOT;
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
-    if (ok) T[((size_t)x * N + l) * ZT + j] = lds[x * NSLOT + c];
+    if (ok) T[((size_t)x * N + l) * ZT + zcol(j, KZ)] = lds[x * NSLOT + c];
   }
 }
 
@@ -938,7 +945,7 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
     const int pos = dit_iperm<C, C::NS>(n);
     double2 vals[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) vals[t] = T[t * tstride + line + j];
+    for (int t = 0; t < NT; ++t) vals[t] = T[t * tstride + line + zcol(j, KZ)];
 #pragma unroll
     for (int t = 0; t < NT; ++t) lds[pos * NT + t] = vals[t];
   }
@@ -966,8 +973,8 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
   fft_lds_ct<C, 2, 2, 1, NTH, -1>(lds, tw);
   for (int j = threadIdx.x; j < Z; j += NTH) {
     const int n = (j <= KZ) ? j : P - (Z - j);
-    V[line + j] = lds[n * 2];
-    V[vstride + line + j] = lds[n * 2 + 1];
+    V[line + zcol(j, KZ)] = lds[n * 2];
+    V[vstride + line + zcol(j, KZ)] = lds[n * 2 + 1];
   }
 }
 
@@ -1042,7 +1049,7 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
       int j = -1;
       if (n <= KZ) j = n;
       else if (n >= P - nneg) j = n - P + Z;
-      v[n2] = (j >= 0 && l < N) ? ld_nt(&T[t * tstride + line + j]) : make_double2(0.0, 0.0);
+      v[n2] = (j >= 0 && l < N) ? ld_nt(&T[t * tstride + line + zcol(j, KZ)]) : make_double2(0.0, 0.0);
     }
     if constexpr (PZ) dft16_pz<+1>(v);
     else dft16<+1>(v);
@@ -1198,7 +1205,7 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
         int j = -1;
         if (b <= KZ) j = b;
         else if (b >= P - nneg) j = b - P + Z;
-        if (j >= 0) st_nt(&V[fo * vstride + line + j], u4[q]);
+        if (j >= 0) st_nt(&V[fo * vstride + line + zcol(j, KZ)], u4[q]);
       }
     }
   }
@@ -1230,7 +1237,7 @@ __global__ __launch_bounds__(256) void k_fft_x_fwd_t(const double2* __restrict__
     bool ok;
     const int j = slot_col(c, q0, Z, KZ, &ok);
     vv[k] = make_double2(0.0, 0.0);
-    if (w < P * NSLOT && ok) vv[k] = Vt[((size_t)dit_perm<C, C::NS>(pos) * N + l) * ZT + j];
+    if (w < P * NSLOT && ok) vv[k] = Vt[((size_t)dit_perm<C, C::NS>(pos) * N + l) * ZT + zcol(j, KZ)];
   }
 #pragma unroll
   for (int k = 0; k < IT; ++k) {
@@ -2393,7 +2400,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   p->Z = Z;
   p->K = M / 2;
   p->KZ = Z / 2;
-  p->ZT = (Z + 7) & ~7;
+  p->ZT = (Z + ZCOL_SHIFT + 7) & ~7;  // row pitch of the transformed fields (zcol)
   p->Ka = p->K + 1;
   p->Kb = p->KZ + 1;
   p->NT = (N + 15) / 16;
