@@ -242,6 +242,7 @@ struct YparArgs {
   double* out;
   int M, N, Z;
   long comp_stride;  // doubles between components
+  int acc;           // 1: out += the product (the adjoint's D^T accumulation); 0: out = the product
 };
 
 // Block bx of 32 flattened columns, component bz.  Waves 0, 1 compute the
@@ -321,8 +322,13 @@ __device__ __forceinline__ void ypar_body(const YparArgs& G, int bx, int bz, dou
       if (t >= T || h >= G.nEf) continue;
       const double e = acc[t][r];
       const double o = (h < G.nOf) ? sO[h * 32 + lc] : 0.0;
-      G.out[base + (size_t)h * C2] = e + o;
-      if (N - 1 - h != h) G.out[base + (size_t)(N - 1 - h) * C2] = e - o;
+      if (G.acc) {
+        G.out[base + (size_t)h * C2] += e + o;
+        if (N - 1 - h != h) G.out[base + (size_t)(N - 1 - h) * C2] += e - o;
+      } else {
+        G.out[base + (size_t)h * C2] = e + o;
+        if (N - 1 - h != h) G.out[base + (size_t)(N - 1 - h) * C2] = e - o;
+      }
     }
   }
 }
@@ -1767,6 +1773,8 @@ struct jsd_plan {
   double *DF1, *DF2, *DFT1, *D_R, *D2_R, *RB_R, *LI_R, *PO_R;  // adjoint: D^T tiles, row-major tables
   double *YF1, *YF2;  // parity-block fragments of D and D^2 for k_ypar
   YparArgs yp1, yp2;  // their block geometry
+  YparArgs yp1t;      // D^T in the same parity form (adjoint plans; the batched D^T accumulation)
+  double* YF1T;
   double *YS1, *YS2;  // the same operators in k_ypar_stream's layout
   double* YS1T;       // D^T in that layout (adjoint plans): u_bar += D^T Du_bar
   double* TF;         // eigen-form adjoint plans: transposed solve blocks (k_solve_T8p)
@@ -2328,6 +2336,7 @@ static void ypar_frags(const double* A, int N, int sgn, std::vector<double>& F, 
   G.ET = (nEf + 15) / 16;
   G.OT = (nOf + 15) / 16;
   G.anti = sgn < 0 ? 1 : 0;
+  G.acc = 0;
   G.KIE = G.anti ? G.KO : G.KE;
   G.KIO = G.anti ? G.KE : G.KO;
   F.assign((size_t)(G.ET * G.KIE + G.OT * G.KIO) * 64, 0.0);
@@ -2577,6 +2586,8 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     tile_D(DT.data(), N, p->NT, p->KS, tiled);
     TRY(upload(p, &p->DFT1, tiled.data(), tiled.size()));
     TRY(ypar_stream_setup_T(p, DT.data()));
+    ypar_frags(DT.data(), N, -1, tiled, p->yp1t);  // D^T is centro-antisymmetric like D
+    TRY(upload(p, &p->YF1T, tiled.data(), tiled.size()));
     TRY(upload(p, &p->D_R, d->D, nn));
     TRY(upload(p, &p->D2_R, D2.data(), nn));
     TRY(upload(p, &p->RB_R, d->rhs_base, 3 * nn));
@@ -2692,6 +2703,7 @@ static YparArgs ypar_args(const jsd_plan* p, const void* in, void* out, int orde
   G.N = p->N;
   G.Z = p->Z;
   G.comp_stride = (long)(2 * p->fs);
+  G.acc = 0;
   const long ncol = (long)p->M * 2 * p->Z;
   *nblk = (unsigned)((ncol + 31) / 32);
   *lds = std::max(*lds, (size_t)0);
@@ -3471,6 +3483,24 @@ static YgemmArgs hvt_args(const jsd_plan* p, const Roles& r) {
 static YgemmArgs acc_args(const jsd_plan* p, double2* UB) {
   return ygemm_args(p, p->DFT1, p->DU, nullptr, nullptr, UB + p->fs, nullptr, nullptr, nullptr, 0L);
 }
+// the same accumulation v_bar += D^T Dv_bar on the parity blocks of D^T (ypar_body)
+static YparArgs acc_ypar_args(const jsd_plan* p, double2* UB, unsigned* nblk) {
+  size_t lds = 0;
+  YparArgs G = ypar_args(p, p->DU, UB + p->fs, 1, nblk, &lds);
+  const YparArgs T = p->yp1t;
+  G.F = p->YF1T;
+  G.ET = T.ET;
+  G.OT = T.OT;
+  G.KIE = T.KIE;
+  G.KIO = T.KIO;
+  G.KE = T.KE;
+  G.KO = T.KO;
+  G.nEf = T.nEf;
+  G.nOf = T.nOf;
+  G.anti = T.anti;
+  G.acc = 1;
+  return G;
+}
 
 // The solve transpose of one substep and the h_v / h_g transpose: one
 // eigen-form kernel writing Hbar (p->H), or the table form and k_ygemm<GEMM_HVT>.
@@ -3875,6 +3905,8 @@ struct jsd_batch {
   bool adj8 = false;                    // adjoint: eigen-form transpose with the h_v / h_g transpose folded in
   BatchStage solveT8[3];
   BatchStage solveT[3], hvt[3], nlT, acc;  // adjoint, indexed by call order c (substep 2 - c)
+  BatchStage accp;                         // the D^T accumulation on parity blocks (accpar)
+  bool accpar = false;
   int cap[BS_NKIND];
   int total[BS_NKIND];
   hipStream_t cs = nullptr;
@@ -3950,7 +3982,10 @@ static int batch_bwd_step(const jsd_batch* b, hipStream_t st) {
       if (!rc) rc = launch_stage<OpYgemmHVT>(b, b->hvt[c], b->cap[BS_HVT], st);
     }
     if (!rc) rc = launch_stage<OpNlLinT>(b, b->nlT, b->cap[BS_NLT], st);
-    if (!rc) rc = launch_stage<OpYgemmAcc>(b, b->acc, b->cap[BS_ACC], st);
+    if (!rc) {
+      if (b->accpar) rc = launch_stage<OpYpar>(b, b->accp, b->cap[BS_ACC], st);
+      else rc = launch_stage<OpYgemmAcc>(b, b->acc, b->cap[BS_ACC], st);
+    }
     if (rc) return rc;
   }
   return JSD_OK;
@@ -4220,6 +4255,18 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
       }
       BTRY(make_stage(b, b->nlT, nta, ng));
       BTRY(make_stage(b, b->acc, aa, yg));
+      b->accpar = !getenv("JSD_BATCH_ACC_GEMM");
+      for (int r = 0; r < nrun; ++r) b->accpar = b->accpar && b->plans[r]->YF1T != nullptr;
+      if (b->accpar) {
+        std::vector<YparArgs> pa;
+        std::vector<dim3> pg;
+        for (int r = 0; r < nrun; ++r) {
+          unsigned nyb = 0;
+          pa.push_back(acc_ypar_args(b->plans[r], b->adj[r], &nyb));
+          pg.push_back(dim3(nyb));
+        }
+        BTRY(make_stage(b, b->accp, pa, pg));
+      }
       for (int c = 0; c < 3; ++c) {
         const int s = 2 - c;
         std::vector<SolveTArgs> ta;
@@ -4252,7 +4299,7 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
   b->total[BS_SOLVET] = b->vjp ? (b->adj8 ? b->solveT8[0].total : b->solveT[0].total) : 0;
   b->total[BS_HVT] = b->vjp && !b->adj8 ? b->hvt[0].total : 0;
   b->total[BS_NLT] = b->vjp ? b->nlT.total : 0;
-  b->total[BS_ACC] = b->vjp ? b->acc.total : 0;
+  b->total[BS_ACC] = b->vjp ? (b->accpar ? b->accp.total : b->acc.total) : 0;
   for (int k = 0; k < BS_NKIND; ++k) {
     b->cap[k] = std::max(1, b->total[k]);
     b->tile[k] = b->waves[k] = 0;
@@ -4300,7 +4347,8 @@ extern "C" int jsd_batch_create(jsd_plan* const* plans, void* const* states, voi
           tune_stage<OpYgemmHVT>(b, BS_HVT, b->hvt, 3, st);
         }
         tune_stage<OpNlLinT>(b, BS_NLT, &b->nlT, 1, st);
-        tune_stage<OpYgemmAcc>(b, BS_ACC, &b->acc, 1, st);
+        if (b->accpar) tune_stage<OpYpar>(b, BS_ACC, &b->accp, 1, st);
+        else tune_stage<OpYgemmAcc>(b, BS_ACC, &b->acc, 1, st);
       }
     }
     off = 0;
