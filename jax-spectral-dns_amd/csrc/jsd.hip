@@ -564,11 +564,12 @@ __device__ __forceinline__ int xcd_logical() {
   return (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
 }
 
-enum { OP_ID = 0, OP_DX = 1, OP_DZ = 2 };
+enum { OP_ID = 0, OP_DX = 1, OP_DZ = 2, OP_ZERO = 3 };
 
 __device__ __forceinline__ double2 apply_op(int op, double2 v, double kxv, double kzv) {
   if (op == OP_DX) return cik(kxv, v);
   if (op == OP_DZ) return cik(kzv, v);
+  if (op == OP_ZERO) return make_double2(0.0, 0.0);
   return v;
 }
 
@@ -645,18 +646,67 @@ enum { PROD_NS = 0, PROD_PERT = 1, PROD_DUAL = 2 };
 // row[2c+1] = (d_y u_c, d_z u_c); the dual adds the forward field in rows
 // 6..11 (the base flow U(y) is added here).  H gets the three components.
 // row[t * ST] holds packed transform t at one physical point.
-template <int PROD, int ST = 1>
+// The fields at a physical point from the six inverse rows (row t at
+// row[t * ST]): (u_c, d_x u_c), (d_y u_c, d_z u_c) for c = 0, 1, 2, or with
+// T5 (u_0, u_1), (d_z u_0, d_z u_1), (u_2, d_z u_2), (d_x u_0, d_x u_1),
+// (d_x u_2, d_y u_0), (d_y u_1, d_y u_2) (zrow_src, zrow_post).
+template <int ST, bool T5>
+__device__ __forceinline__ void unpack_fields(const double2* row, double* u, double (*g)[3]) {
+  if constexpr (T5) {
+    const double2 r0 = row[0], r1 = row[ST], r2 = row[2 * ST], r3 = row[3 * ST], r4 = row[4 * ST], r5 = row[5 * ST];
+    u[0] = r0.x;
+    u[1] = r0.y;
+    g[0][2] = r1.x;
+    g[1][2] = r1.y;
+    u[2] = r2.x;
+    g[2][2] = r2.y;
+    g[0][0] = r3.x;
+    g[1][0] = r3.y;
+    g[2][0] = r4.x;
+    g[0][1] = r4.y;
+    g[1][1] = r5.x;
+    g[2][1] = r5.y;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2 r0 = row[(2 * c) * ST];
+      const double2 r1 = row[(2 * c + 1) * ST];
+      u[c] = r0.x;
+      g[c][0] = r0.y;
+      g[c][1] = r1.x;
+      g[c][2] = r1.y;
+    }
+  }
+}
+
+// The five-row layout (T5) of the transformed fields of one velocity:
+// T0 = u_0 + i u_1, T1 = u_2 (+ i 0), T2 = d_x u_0 + i d_x u_1,
+// T3 = d_x u_2 + i d_y u_0, T4 = d_y u_1 + i d_y u_2.  The z derivatives come
+// from the z spectra: inverse row 1 = i kz T0 (d_z u_0 + i d_z u_1), inverse
+// row 2 = (1 - kz) T1 = T1 + i (i kz T1) (u_2 + i d_z u_2, as T1's imaginary
+// part is zero), so the six inverse rows are T0, i kz T0, (1 - kz) T1, T2,
+// T3, T4: every lane forms its row from one load, with no other lane's data.
+// The wavenumber of stored index j is kz1 times its signed index.
+// (zrow_src gives the row of T that inverse row t reads; zrow_post applies
+// the kz factor once all loads of the thread are in flight, so that no load
+// waits inside the load loop.)
+template <bool T5>
+__device__ __forceinline__ int zrow_src(int t) {
+  return T5 ? (t <= 1 ? 0 : t - 1) : t;
+}
+template <bool T5>
+__device__ __forceinline__ void zrow_post(int t, int j, int Z, int KZ, double kz1, double2& a) {
+  if constexpr (T5) {
+    if (j < 0 || t > 2 || t == 0) return;
+    const double k = kz1 * (double)(j <= KZ ? j : j - Z);
+    a = t == 1 ? make_double2(-k * a.y, k * a.x) : make_double2((1.0 - k) * a.x, (1.0 - k) * a.y);
+  }
+}
+
+template <int PROD, int ST = 1, bool T5 = false>
 __device__ __forceinline__ void zmid_products(const double2* row, double U, double dU, double clin, double* H) {
   double u[3], g[3][3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double2 r0 = row[(2 * c) * ST];
-    const double2 r1 = row[(2 * c + 1) * ST];
-    u[c] = r0.x;
-    g[c][0] = r0.y;
-    g[c][1] = r1.x;
-    g[c][2] = r1.y;
-  }
+  unpack_fields<ST, T5>(row, u, g);
   const double div = g[0][0] + g[1][1] + g[2][2];
   if constexpr (PROD == PROD_NS) {
     // H = -(u.grad) u + 0.5 (div u) u   (navier_stokes.py:236-261)
@@ -1004,11 +1054,12 @@ __host__ __device__ constexpr int z256_threads() {
 
 // PZ (P = 256 only): at most 129 stored kz (|kz| <= 64), so step 1 of the
 // inverse sees zeros at n2 = 5..11 (dft16_pz).
-template <int P, int NT, int PROD, bool PZ, int LN = 1>
+template <int P, int NT, int PROD, bool PZ, int LN = 1, bool T5 = false>
 __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void k_fft_z_mid_r(
     const double2* __restrict__ T, long tstride, double2* __restrict__ V, long vstride,
     const double2* __restrict__ tw, int N, int Z, int ZT, double clin, const double* __restrict__ Ub,
-    const double* __restrict__ dUb) {
+    const double* __restrict__ dUb, double kz1) {
+  static_assert(!T5 || NT == 6, "the five-row layout holds one velocity");
   static_assert(P == 128 || P == 256, "register z pass: P = 128 or 256");
   static_assert(!PZ || P == 256, "the pruned first stage assumes P = 256");
   constexpr int N1 = P / 16;
@@ -1049,13 +1100,22 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
     const int t = r / N1, n1 = r % N1;
     const int l = l0 + li;
     const size_t line = ((size_t)x * N + (l < N ? l : 0)) * ZT;
+    const int ts = zrow_src<T5>(t);
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
       const int n = n1 + N1 * n2;
       int j = -1;
       if (n <= KZ) j = n;
       else if (n >= P - nneg) j = n - P + Z;
-      v[n2] = (j >= 0 && l < N) ? ld_nt(&T[t * tstride + line + zcol(j, KZ)]) : make_double2(0.0, 0.0);
+      v[n2] = (j >= 0 && l < N) ? ld_nt(&T[ts * tstride + line + zcol(j, KZ)]) : make_double2(0.0, 0.0);
+    }
+    if constexpr (T5) {
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) {
+        const int n = n1 + N1 * n2;
+        const int j = n <= KZ ? n : (n >= P - nneg ? n - P + Z : -1);
+        zrow_post<T5>(t, j, Z, KZ, kz1, v[n2]);
+      }
     }
     if constexpr (PZ) dft16_pz<+1>(v);
     else dft16<+1>(v);
@@ -1101,7 +1161,7 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
     const int l = l0 + li < N ? l0 + li : N - 1;
     const double U = Ub ? Ub[l] : 0.0;
     const double dU = dUb ? dUb[l] : 0.0;
-    if (zz < LN * P) zmid_products<PROD, P>(lds + li * REG + z, U, dU, clin, Hs[q]);
+    if (zz < LN * P) zmid_products<PROD, P, T5>(lds + li * REG + z, U, dU, clin, Hs[q]);
   }
   __syncthreads();
   // packed rows (H0 + i H1) and (H2 + 0 i) at [o][z]
@@ -1787,6 +1847,9 @@ struct jsd_plan {
   int ys_grid[2];     // persistent grid of k_ypar_stream for D, D^2 (0: k_ypar instead)
   int zreg;           // 1: register z passes for lengths 128 and 256 (default); 0: the LDS kernels
                       // (JSD_ZPASS=lds, a diagnostic switch)
+  double kz1;         // the z wavenumber of stored index 1
+  int t5;             // NS / perturbation with a register z pass: five packed x transforms (the z pass
+                      // forms d_z u_c itself); 0 under JSD_T5=0 (a diagnostic switch)
   double *VA, *INFL;                                           // adjoint: v_a, inverse influence matrices
   int adjoint;         // the discrete-adjoint tables are present
   double2* SS;         // adjoint of a nonlinear step: the three recomputed substep states
@@ -2615,6 +2678,7 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   }
   TRY(upload(p, &p->kx, d->kx, M));
   TRY(upload(p, &p->kz, d->kz, Z));
+  p->kz1 = Z > 1 ? d->kz[1] : 0.0;
   TRY(upload(p, &p->wy, d->y_weights, N));
   if (d->kind != JSD_KIND_NS) {
     TRY(upload(p, &p->Ub, d->base_u, N));
@@ -2624,6 +2688,11 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
   }
   TRY(make_fft(p, d->ppx, &p->fpx, &p->permx, &p->ipermx, &p->twx));
   TRY(make_fft(p, d->ppz, &p->fpz, &p->permz, &p->ipermz, &p->twz));
+  {
+    const char* env5 = getenv("JSD_T5");
+    p->t5 = !p->generic_fft && p->zreg && (plan_is<RZ128>(p->fpz) || plan_is<RZ256>(p->fpz)) &&
+            d->kind != JSD_KIND_DUAL && !(env5 && env5[0] == '0');
+  }
   const size_t fb = p->fs * sizeof(double2);
   TRY(dalloc(p, (void**)&p->DU, 3 * fb));
   p->HGC = nullptr;
@@ -2786,11 +2855,25 @@ static bool xset_alternates(const XInvSet& set, int ntr) {
   return true;
 }
 
+// the five-row set of fill_set_u with t5
+static bool xset_t5(const XInvSet& set, int ntr) {
+  if (ntr != 5) return false;
+  const int oA[5] = {OP_ID, OP_ID, OP_DX, OP_DX, OP_ID};
+  const int oB[5] = {OP_ID, OP_ZERO, OP_DX, OP_ID, OP_ID};
+  for (int t = 0; t < 5; ++t)
+    if (set.opA[t] != oA[t] || set.opB[t] != oB[t]) return false;
+  return true;
+}
+
 static void launch_x_inv_ct(jsd_plan* p, const XInvSet& set, int ntr, int nqb, dim3 grid, size_t lds, hipStream_t st,
                             bool* done) {
   const bool reg = p->xreg && xset_alternates(set, ntr);
+  const bool reg5 = p->xreg && xset_t5(set, ntr);
   *done = visit_x(p->fpx, [&](auto c) {
-    if (reg && p->xreg == 2)
+    if (reg5)
+      hipLaunchKernelGGL((k_fft_x_inv_r<decltype(c), false, true>), grid, dim3(256), lds, st, set, ntr, nqb,
+                         p->fpx.tw, p->kx, p->kz, p->M, p->N, p->Z, p->ZT, p->s_inv);
+    else if (reg && p->xreg == 2)
       hipLaunchKernelGGL((k_fft_x_inv_r<decltype(c), true>), grid, dim3(256), lds, st, set, ntr, nqb, p->fpx.tw, p->kx,
                          p->kz, p->M, p->N, p->Z, p->ZT, p->s_inv);
     else if (reg)
@@ -2800,6 +2883,39 @@ static void launch_x_inv_ct(jsd_plan* p, const XInvSet& set, int ntr, int nqb, d
       hipLaunchKernelGGL(k_fft_x_inv_t<decltype(c)>, grid, dim3(256), lds, st, set, ntr, nqb, p->fpx.tw, p->kx, p->kz,
                          p->M, p->N, p->Z, p->ZT, p->s_inv);
   });
+}
+
+// The packed inverse x transforms of a velocity U (D U in DU) from transform
+// t0 on: six rows (u_c, d_x u_c), (D u_c, d_z u_c), or with t5 the five rows
+// (u_0, u_1), (u_2, 0), (d_x u_0, d_x u_1), (d_x u_2, D u_0), (D u_1, D u_2),
+// whose z pass forms the z derivatives from the z spectra of the first two
+// (zrow_src, zrow_post).  Returns the number of rows.
+static int fill_set_u(const jsd_plan* p, const double2* U, const double2* DU, XInvSet& set, int t0, bool t5) {
+  const size_t fs = p->fs;
+  if (t5) {
+    const double2* A[5] = {U, U + 2 * fs, U, U + 2 * fs, DU + fs};
+    const double2* B[5] = {U + fs, U + 2 * fs, U + fs, DU, DU + 2 * fs};
+    const int oA[5] = {OP_ID, OP_ID, OP_DX, OP_DX, OP_ID};
+    const int oB[5] = {OP_ID, OP_ZERO, OP_DX, OP_ID, OP_ID};
+    for (int t = 0; t < 5; ++t) {
+      set.A[t0 + t] = A[t];
+      set.B[t0 + t] = B[t];
+      set.opA[t0 + t] = oA[t];
+      set.opB[t0 + t] = oB[t];
+    }
+    return 5;
+  }
+  for (int c = 0; c < 3; ++c) {
+    set.A[t0 + 2 * c] = U + c * fs;
+    set.opA[t0 + 2 * c] = OP_ID;
+    set.B[t0 + 2 * c] = U + c * fs;
+    set.opB[t0 + 2 * c] = OP_DX;
+    set.A[t0 + 2 * c + 1] = DU + c * fs;
+    set.opA[t0 + 2 * c + 1] = OP_ID;
+    set.B[t0 + 2 * c + 1] = U + c * fs;
+    set.opB[t0 + 2 * c + 1] = OP_DZ;
+  }
+  return 6;
 }
 
 static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStream_t st, bool have_du) {
@@ -2840,27 +2956,12 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     if (rc) return rc;
   }
   XInvSet set;
-  for (int c = 0; c < 3; ++c) {
-    // (u_c, d_x u_c) and (d_y u_c, d_z u_c)
-    set.A[2 * c] = U + c * fs;
-    set.opA[2 * c] = OP_ID;
-    set.B[2 * c] = U + c * fs;
-    set.opB[2 * c] = OP_DX;
-    set.A[2 * c + 1] = p->DU + c * fs;
-    set.opA[2 * c + 1] = OP_ID;
-    set.B[2 * c + 1] = U + c * fs;
-    set.opB[2 * c + 1] = OP_DZ;
-    if (ntr == 12) {
-      set.A[6 + 2 * c] = UF + c * fs;
-      set.opA[6 + 2 * c] = OP_ID;
-      set.B[6 + 2 * c] = UF + c * fs;
-      set.opB[6 + 2 * c] = OP_DX;
-      set.A[7 + 2 * c] = p->DWF + c * fs;
-      set.opA[7 + 2 * c] = OP_ID;
-      set.B[7 + 2 * c] = UF + c * fs;
-      set.opB[7 + 2 * c] = OP_DZ;
-    }
-  }
+  // (u_c, d_x u_c) and (d_y u_c, d_z u_c), five rows with t5; the dual's
+  // forward field W in six more
+  const bool t5 = p->t5 && ntr == 6;
+  fill_set_u(p, U, p->DU, set, 0, t5);
+  if (ntr == 12) fill_set_u(p, UF, p->DWF, set, 6, false);
+  if (t5) ntr = 5;
   for (int t = 0; t < ntr; ++t) set.T[t] = p->T + t * ts;
   const bool ct = !p->generic_fft;
   prof_begin(p, 1, st);
@@ -2880,15 +2981,21 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     auto launch = [&](auto pc, auto pz) {
       constexpr int P = decltype(pc)::value;
       constexpr bool PZ = decltype(pz)::value;
-      if (prod == PROD_NS)
+      if (prod == PROD_NS && t5)
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_NS, PZ, 2, true>), grid2, dim3(z256_threads<6, 2>()), 0, st,
+                           p->T, (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
+      else if (prod == PROD_PERT && t5)
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_PERT, PZ, 2, true>), grid2, dim3(z256_threads<6, 2>()), 0, st,
+                           p->T, (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
+      else if (prod == PROD_NS)
         hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_NS, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
-                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
       else if (prod == PROD_PERT)
         hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_PERT, PZ, 2>), grid2, dim3(z256_threads<6, 2>()), 0, st, p->T,
-                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
       else
         hipLaunchKernelGGL((k_fft_z_mid_r<P, 12, PROD_DUAL, PZ>), grid, dim3(z256_threads<12>()), 0, st, p->T,
-                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+                           (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
     };
     if (z_128)
       launch(std::integral_constant<int, 128>(), std::false_type());
@@ -3575,18 +3682,8 @@ static int forward_inverse_x(jsd_plan* p, const double2* U, hipStream_t st) {
   const size_t ts = (size_t)p->d.ppx * N * ZT;
   const size_t lds_x = (size_t)p->d.ppx * NSLOT * sizeof(double2);
   XInvSet set;
-  for (int c = 0; c < 3; ++c) {
-    set.A[2 * c] = U + c * fs;
-    set.opA[2 * c] = OP_ID;
-    set.B[2 * c] = U + c * fs;
-    set.opB[2 * c] = OP_DX;
-    set.A[2 * c + 1] = p->DU + c * fs;
-    set.opA[2 * c + 1] = OP_ID;
-    set.B[2 * c + 1] = U + c * fs;
-    set.opB[2 * c + 1] = OP_DZ;
-  }
-  for (int t = 0; t < 6; ++t) set.T[t] = p->T + t * ts;
-  const int ntr = 6;
+  const int ntr = fill_set_u(p, U, p->DU, set, 0, p->t5 != 0);
+  for (int t = 0; t < ntr; ++t) set.T[t] = p->T + t * ts;
   const dim3 grid_xi(8 * ((ntr * nqb * N + 7) / 8));
   bool ok = false;
   launch_x_inv_ct(p, set, ntr, nqb, grid_xi, lds_x, st, &ok);
@@ -3643,12 +3740,18 @@ static int nl_adjoint_terms(jsd_plan* p, double2* UB, const double2* Us, double2
       auto launch = [&](auto pc, auto pz) {
         constexpr int P = decltype(pc)::value;
         constexpr bool PZ = decltype(pz)::value;
-        if (prod == PROD_NS)
+        if (prod == PROD_NS && p->t5)
+          hipLaunchKernelGGL((k_adj_z_r<P, PROD_NS, PZ, 2, true>), grid2, dim3(z256_threads<8, 2>()), 0, st, p->T,
+                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
+        else if (p->t5)
+          hipLaunchKernelGGL((k_adj_z_r<P, PROD_PERT, PZ, 2, true>), grid2, dim3(z256_threads<8, 2>()), 0, st, p->T,
+                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
+        else if (prod == PROD_NS)
           hipLaunchKernelGGL((k_adj_z_r<P, PROD_NS, PZ, 2>), grid2, dim3(z256_threads<8, 2>()), 0, st, p->T,
-                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
         else
           hipLaunchKernelGGL((k_adj_z_r<P, PROD_PERT, PZ, 2>), grid2, dim3(z256_threads<8, 2>()), 0, st, p->T,
-                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb);
+                             (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
       };
       if (plan_is<RZ128>(p->fpz))
         launch(std::integral_constant<int, 128>(), std::false_type());
