@@ -48,7 +48,8 @@ def algorithmic_bytes(g, nsub: int = 3) -> dict:
     """Bytes one launch of each kernel class must move (inputs read once,
     outputs written once), averaged over the substeps of a step.
     S = one complex128 spectral field, T = one complex128 x-transformed
-    intermediate (ppx x Ny x Nz_modes).  The solve reads the shared
+    intermediate (ppx x Ny x Nz_modes); the nonlinear step keeps five of them
+    (u0 + i u1, u2, the x and y derivatives in three more).  The solve reads the shared
     eigenvector blocks E, E^-1 of the interior D^2 halves (2 (nE^2 + nO^2)
     doubles, once), the v_a parity tables (4 x 64 + 2 doubles per |k|) and
     12.33 S of state: u, v, w, H, on average 4/3 old-term fields; it writes
@@ -62,8 +63,9 @@ def algorithmic_bytes(g, nsub: int = 3) -> dict:
     tables = 2.0 * (nE * nE + nO * nO) * 8.0 + nk * (4 * 64 + 2) * 8.0
     return {
         "ygemm": 6 * S,  # read 3 components, write 3
-        "fft_x_inv": 6 * S + 6 * T,  # read u_c and D u_c, write 6 packed transforms
-        "fft_z_mid": 8 * T,  # read 6 T, write 2 T
+        # five packed rows: (u0, u1), (u2, 0), (dx u0, dx u1), (dx u2, D u0), (D u1, D u2)
+        "fft_x_inv": 6 * S + 5 * T,  # read u_c and D u_c, write 5 packed transforms
+        "fft_z_mid": 7 * T,  # read 5 T, write 2 T (the z derivatives are formed from the z spectra)
         "fft_x_fwd": 2 * T + 3 * S,  # read 2 T, write 3 components of H
         "ygemm_hv": 3 * S + 2 * S,  # read H, write h_v, h_g
         "solve": tables + (3 + 3 + 2.0 * (nsub - 1) / nsub + 3 + 2 + 3.0 * (nsub - 1) / nsub) * S,
