@@ -2693,6 +2693,12 @@ extern "C" int jsd_plan_create(const jsd_plan_desc* d, jsd_plan** out) {
     p->t5 = !p->generic_fft && p->zreg && (plan_is<RZ128>(p->fpz) || plan_is<RZ256>(p->fpz)) &&
             d->kind != JSD_KIND_DUAL && !(env5 && env5[0] == '0');
   }
+  // the five-row z pass forms kz of stored index j as kz1 times its signed
+  // index: keep it to tables of that form (every table the library builds)
+  for (int j = 0; j < Z && p->t5; ++j) {
+    const double want = p->kz1 * (double)(j <= Z / 2 ? j : j - Z);
+    if (fabs(d->kz[j] - want) > 1e-12 * fabs(p->kz1) * Z) p->t5 = 0;
+  }
   const size_t fb = p->fs * sizeof(double2);
   TRY(dalloc(p, (void**)&p->DU, 3 * fb));
   p->HGC = nullptr;
