@@ -36,7 +36,7 @@ CLASSES = ("ygemm", "fft_x_inv", "fft_z_mid", "fft_x_fwd", "ygemm_hv", "solve", 
 KERNEL_NAMES = {
     "ygemm": "k_ypar_stream<true, true, false>",
     "fft_x_inv": "k_fft_x_inv_r<Rad<8, 5, 5, 1>, false, true>",
-    "fft_z_mid": "k_fft_z_mid_r<256, 6, 0, true, 2, true>",
+    "fft_z_mid": "k_fft_z_mid_r<256, 6, 0, true, 1, true>",
     "fft_x_fwd": "k_fft_x_fwd_r<Rad<8, 5, 5, 1> >",
     "ygemm_hv": "k_ygemm<1>",
     "solve": "k_solve_eig8p",

@@ -2987,7 +2987,15 @@ static int nonlinear_H(jsd_plan* p, const double2* U, const double2* UF, hipStre
     auto launch = [&](auto pc, auto pz) {
       constexpr int P = decltype(pc)::value;
       constexpr bool PZ = decltype(pz)::value;
-      if (prod == PROD_NS && t5)
+      // five-row layout: one y line per 256-point workgroup (six two-wave
+      // workgroups per CU measured 0.6 % faster than three of two lines)
+      if (P == 256 && prod == PROD_NS && t5)
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_NS, PZ, 1, true>), grid, dim3(z256_threads<6, 1>()), 0, st,
+                           p->T, (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
+      else if (P == 256 && prod == PROD_PERT && t5)
+        hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_PERT, PZ, 1, true>), grid, dim3(z256_threads<6, 1>()), 0, st,
+                           p->T, (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
+      else if (prod == PROD_NS && t5)
         hipLaunchKernelGGL((k_fft_z_mid_r<P, 6, PROD_NS, PZ, 2, true>), grid2, dim3(z256_threads<6, 2>()), 0, st,
                            p->T, (long)ts, p->V, (long)ts, p->fpz.tw, N, Z, ZT, clin, p->Ub, p->dUb, p->kz1);
       else if (prod == PROD_PERT && t5)
