@@ -1044,7 +1044,9 @@ __global__ __launch_bounds__(zmid_threads(C::P)) void k_fft_z_mid_t(const double
 // transform of the two packed product rows takes the same split, each DFT
 // spread over lanes (below), and stores only the kept bins.  A workgroup
 // takes LN neighbouring y lines of one x: with LN = 2 and 6 rows the 16-lane
-// steps fill 3 of 4 waves.
+// steps fill 3 of 4 waves.  T5: the six inverse rows are formed from the
+// five-row layout of the transformed fields (zrow_src, zrow_post); kz1 is the
+// z wavenumber of stored index 1.
 #define Z16_LD 17  // padded row (in double2) of the [t][n1][k2] exchange layout
 
 template <int NT, int LN = 1>
