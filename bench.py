@@ -258,7 +258,9 @@ def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
     initial energies log-spaced in [1e-6, 1e-2] (seeded fields), each
     gradient over nsteps steps with a checkpoint every 10 and recompute;
     one run at a time (vjp.gain_and_grad) against the concurrent batch
-    (batch.BatchGradient), plus the per-step costs of one run."""
+    (batch.BatchGradient), plus the per-step costs of one run, and the same
+    gradient with every step-start state kept in HBM (checkpoint_every=1, no
+    chunk recompute)."""
     import torch
 
     from jax_spectral_dns_amd import KIND_PERTURBATION
@@ -291,6 +293,12 @@ def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
     J, grad = gain_and_grad(u0, plan, nsteps, checkpoint_every=10)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    # every step-start state kept in HBM (48.6 MB each here): no chunk recompute
+    t0 = time.perf_counter()
+    J1, grad1 = gain_and_grad(u0, plan, nsteps, checkpoint_every=1)
+    torch.cuda.synchronize()
+    t_grad1 = time.perf_counter() - t0
+    stored_diff = float(torch.linalg.norm(grad1 - grad) / torch.linalg.norm(grad))
     # the batch: one run at a time, then concurrently
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -316,6 +324,9 @@ def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
         "forward_ms_per_step": 1e3 * t_fwd / nsteps,
         "gradient_eval_s": t_grad,
         "adjoint_ms_per_step_incl_recompute": 1e3 * (t_grad - t_fwd) / nsteps,
+        "adjoint_ms_per_step_stored_states": 1e3 * (t_grad1 - t_fwd) / nsteps,
+        "gradient_eval_s_stored_states": t_grad1,
+        "max_rel_diff_grad_stored_vs_recompute": stored_diff,
         "gain": J,
         "grad_finite": finite,
         "grad_evals_per_s_batched": nrun / t_bat,
