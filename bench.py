@@ -260,7 +260,8 @@ def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
     one run at a time (vjp.gain_and_grad) against the concurrent batch
     (batch.BatchGradient), plus the per-step costs of one run, and the same
     gradient with every step-start state kept in HBM (checkpoint_every=1, no
-    chunk recompute)."""
+    chunk recompute) and with every step recorded (checkpoint_every=0, no
+    recompute at all)."""
     import torch
 
     from jax_spectral_dns_amd import KIND_PERTURBATION
@@ -299,6 +300,12 @@ def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
     torch.cuda.synchronize()
     t_grad1 = time.perf_counter() - t0
     stored_diff = float(torch.linalg.norm(grad1 - grad) / torch.linalg.norm(grad))
+    # every step recorded (start state, substep states, kept transforms): no recompute at all
+    t0 = time.perf_counter()
+    J0, grad0 = gain_and_grad(u0, plan, nsteps, checkpoint_every=0)
+    torch.cuda.synchronize()
+    t_grad0 = time.perf_counter() - t0
+    recorded_diff = float(torch.linalg.norm(grad0 - grad) / torch.linalg.norm(grad))
     # the batch: one run at a time, then concurrently
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -327,6 +334,9 @@ def adjoint_bench(nsteps: int, nrun: int = 8) -> dict:
         "adjoint_ms_per_step_stored_states": 1e3 * (t_grad1 - t_fwd) / nsteps,
         "gradient_eval_s_stored_states": t_grad1,
         "max_rel_diff_grad_stored_vs_recompute": stored_diff,
+        "adjoint_ms_per_step_recorded": 1e3 * (t_grad0 - t_fwd) / nsteps,
+        "gradient_eval_s_recorded": t_grad0,
+        "max_rel_diff_grad_recorded_vs_recompute": recorded_diff,
         "gain": J,
         "grad_finite": finite,
         "grad_evals_per_s_batched": nrun / t_bat,

@@ -134,6 +134,24 @@ int jsd_step_vjp(jsd_plan* plan, void* grad, int nsteps, void* stream);
  * perturbation plans ignore state and run jsd_step_vjp. */
 int jsd_step_vjp_state(jsd_plan* plan, const void* state, void* grad, double dPdx, void* stream);
 
+/* Recorded steps: the forward keeps what the adjoint of a nonlinear step
+ * would recompute, so that jsd_step_vjp_recorded recomputes nothing.  No
+ * counterpart in the reference, whose reverse-mode AD (jax.value_and_grad
+ * in optimiser.py:115-121) keeps or rematerialises the scan's residuals
+ * itself; this is the same trade made explicit for callers with the memory
+ * (a 96x129x80 step record is 0.42 GB).  Plans as for jsd_step_vjp_state.
+ * jsd_step_record_bytes gives the size of one step's record;
+ * jsd_step_record advances vel_hat by one step (as jsd_step) and writes the
+ * record (device memory of that size: substep states 1 and 2, and the
+ * inverse x transforms of substeps 0 and 1); jsd_step_vjp_recorded is
+ * jsd_step_vjp_state with state the step's start state and record that
+ * step's record.  The adjoint overwrites the record's transforms: one
+ * adjoint per record. */
+int jsd_step_record_bytes(jsd_plan* plan, size_t* bytes);
+int jsd_step_record(jsd_plan* plan, void* vel_hat, void* record, double dPdx, void* stream);
+int jsd_step_vjp_recorded(jsd_plan* plan, const void* state, const void* record, void* grad, double dPdx,
+                          void* stream);
+
 /* Replaces the base flow of a perturbation plan (ny values of U(y) and U'(y),
  * host arrays, copied before return): the time-dependent base flow of
  * NavierStokesVelVortPerturbationInstationary (vel_base_fn,

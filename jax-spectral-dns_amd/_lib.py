@@ -70,6 +70,9 @@ EXPORTS = (
     "jsd_ydiff",
     "jsd_step_vjp",
     "jsd_step_vjp_state",
+    "jsd_step_record_bytes",
+    "jsd_step_record",
+    "jsd_step_vjp_recorded",
     "jsd_energy_grad",
     "jsd_profile",
     "jsd_set_base_flow",
@@ -109,6 +112,9 @@ def load() -> ctypes.CDLL:
     lib.jsd_ydiff.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     lib.jsd_step_vjp.argtypes = [vp, vp, ctypes.c_int, vp]
     lib.jsd_step_vjp_state.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    lib.jsd_step_record_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.jsd_step_record.argtypes = [vp, vp, vp, ctypes.c_double, vp]
+    lib.jsd_step_vjp_recorded.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_energy_grad.argtypes = [vp, vp, vp, ctypes.c_double, vp]
     lib.jsd_profile.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
     lib.jsd_set_base_flow.argtypes = [vp, _dp, _dp, vp]

@@ -3875,6 +3875,84 @@ extern "C" int jsd_step_vjp_state(jsd_plan* p, const void* state, void* grad, do
   return JSD_OK;
 }
 
+// A step record: the substep states 1 and 2 of one nonlinear step and the
+// inverse x transforms of substeps 0 and 1 (when the compile-time transform
+// kernels run), written by the forward step and read by its adjoint, which
+// then recomputes nothing of the step.  Each substep's slot holds six rows
+// (REC_ROWS) whatever the forward layout: the adjoint writes its six adjoint
+// rows over the kept transforms.
+#define REC_ROWS 6
+static size_t record_elems(const jsd_plan* p) {
+  const size_t ts = (size_t)p->d.ppx * p->N * p->ZT;
+  const size_t rows = p->generic_fft ? 0 : REC_ROWS;
+  return 6 * p->fs + 2 * rows * ts;
+}
+
+static int record_check(const jsd_plan* p) {
+  if (p->d.kind == JSD_KIND_DUAL || (p->d.kind == JSD_KIND_PERTURBATION && p->d.linearise) || !p->adjoint)
+    JSD_FAIL(JSD_ERR_STATE, "step records are for NS and nonlinear perturbation plans created with adjoint = 1");
+  if (p->HGC) JSD_FAIL(JSD_ERR_STATE, "the discrete adjoint has no coupling_term = False transpose");
+  return JSD_OK;
+}
+
+extern "C" int jsd_step_record_bytes(jsd_plan* p, size_t* bytes) {
+  if (!p || !bytes) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  int rc = record_check(p);
+  if (rc) return rc;
+  *bytes = record_elems(p) * sizeof(double2);
+  return JSD_OK;
+}
+
+extern "C" int jsd_step_record(jsd_plan* p, void* vel_hat, void* record, double dPdx, void* stream) {
+  if (!p || !vel_hat || !record) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  int rc = record_check(p);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  double2* U = (double2*)vel_hat;
+  double2* R = (double2*)record;
+  const size_t sb = 3 * p->fs * sizeof(double2);
+  const size_t ts = (size_t)p->d.ppx * p->N * p->ZT;
+  double2* const T0 = p->T;
+  double2* Tk[2] = {nullptr, nullptr};
+  if (!p->generic_fft) {
+    Tk[0] = R + 6 * p->fs;
+    Tk[1] = Tk[0] + REC_ROWS * ts;
+  }
+  // the substeps of jsd_step (the same D u chain), substeps 0 and 1 writing
+  // their inverse x transforms into the record
+  const bool chain = p->EVF != nullptr;
+  for (int s = 0; s < 3; ++s) {
+    if (s < 2 && Tk[s]) p->T = Tk[s];
+    rc = substep(p, U, nullptr, s, dPdx, st, chain && s > 0, chain && s < 2);
+    p->T = T0;
+    if (rc) return rc;
+    if (s < 2) HIPCHK(hipMemcpyAsync(R + 3 * s * p->fs, U, sb, hipMemcpyDeviceToDevice, st));
+  }
+  return JSD_OK;
+}
+
+extern "C" int jsd_step_vjp_recorded(jsd_plan* p, const void* state, const void* record, void* grad, double dPdx,
+                                     void* stream) {
+  if (!p || !state || !record || !grad) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
+  int rc = record_check(p);
+  if (rc) return rc;
+  (void)dPdx;  // the adjoint of a step does not depend on its pressure gradient
+  hipStream_t st = (hipStream_t)stream;
+  const double2* R = (const double2*)record;
+  const size_t ts = (size_t)p->d.ppx * p->N * p->ZT;
+  double2* Tk[3] = {nullptr, nullptr, nullptr};
+  if (!p->generic_fft) {
+    Tk[0] = (double2*)R + 6 * p->fs;
+    Tk[1] = Tk[0] + REC_ROWS * ts;
+  }
+  const double2* Ss[3] = {(const double2*)state, R, R + 3 * p->fs};
+  for (int s = 2; s >= 0; --s) {
+    rc = adjoint_substep_nl(p, (double2*)grad, Ss[s], Tk[s], s, s < 2, st);
+    if (rc) return rc;
+  }
+  return JSD_OK;
+}
+
 extern "C" int jsd_step_vjp(jsd_plan* p, void* grad, int nsteps, void* stream) {
   if (!p || !grad || nsteps < 0) JSD_FAIL(JSD_ERR_ARG, "invalid argument");
   if (p->d.kind != JSD_KIND_PERTURBATION || !p->d.linearise || !p->adjoint)

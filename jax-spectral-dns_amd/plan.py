@@ -128,6 +128,38 @@ class StepPlan:
         _lib.check(_lib.load().jsd_step_vjp(self._h, _dev(grad), int(nsteps), _stream()))
         return grad
 
+    def record_elems(self) -> int:
+        """complex128 elements of one step record (step_record / vjp_recorded)."""
+        n = ctypes.c_size_t(0)
+        _lib.check(_lib.load().jsd_step_record_bytes(self._h, ctypes.byref(n)))
+        return int(n.value) // 16
+
+    def step_record(self, vel_hat: torch.Tensor, record: torch.Tensor, dPdx: float = 0.0) -> torch.Tensor:
+        """One step of vel_hat in place, writing the step's record (a complex128
+        CUDA tensor of record_elems() elements) for vjp_recorded."""
+        self._check_state(vel_hat)
+        self._check_record(record)
+        _lib.check(_lib.load().jsd_step_record(self._h, _dev(vel_hat), _dev(record), float(dPdx), _stream()))
+        return vel_hat
+
+    def vjp_recorded(self, state: torch.Tensor, record: torch.Tensor, grad: torch.Tensor, dPdx: float = 0.0) -> torch.Tensor:
+        """vjp_state about the step that step_record recorded from the start
+        state `state`: no recomputation.  The record is consumed (its
+        transforms are overwritten)."""
+        self._check_state(state)
+        self._check_state(grad)
+        self._check_record(record)
+        _lib.check(
+            _lib.load().jsd_step_vjp_recorded(self._h, _dev(state), _dev(record), _dev(grad), float(dPdx), _stream())
+        )
+        return grad
+
+    def _check_record(self, record: torch.Tensor) -> None:
+        if record.dtype != torch.complex128 or not record.is_cuda or not record.is_contiguous():
+            raise _lib.JsdError("a step record is a contiguous complex128 CUDA tensor")
+        if record.numel() < self.record_elems():
+            raise _lib.JsdError("step record holds %d elements, needs %d" % (record.numel(), self.record_elems()))
+
     def vjp_state(self, state: torch.Tensor, grad: torch.Tensor, dPdx: float = 0.0) -> torch.Tensor:
         """In place: adjoint after one step -> adjoint before it, the step
         taken from the forward state `state` (nonlinear plans built with

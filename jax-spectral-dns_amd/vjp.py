@@ -16,7 +16,10 @@ its transpose needs no forward state and nothing is stored or recomputed.
 ``checkpoint_every=1`` keeps every step-start state (one state of 3 x Nx x Ny
 x Nz complex128 per step, 48.6 MB at 96x129x80, so a 288 GB GPU holds
 thousands) and the backward recomputes no chunk; the substep states inside a
-step are still recomputed by ``jsd_step_vjp_state``.
+step are still recomputed by ``jsd_step_vjp_state``.  ``checkpoint_every=0``
+records every step (``jsd_step_record``: its start state, two substep states
+and the kept transforms, 0.47 GB a step at 96x129x80) and the backward
+recomputes nothing (``jsd_step_vjp_recorded``).
 ``energy(u, plan)`` is the energy functional with its gradient from
 ``jsd_energy_grad``; ``gain_and_grad`` evaluates J = E(u_T)/E(u_0) and
 dJ/du_0.
@@ -47,6 +50,21 @@ class _Run(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0r: torch.Tensor, plan: StepPlan, nsteps: int, checkpoint_every: int, dPdx: float):  # type: ignore[override]
         u = torch.view_as_complex(u0r.contiguous()).clone()
+        ctx.recs = None
+        if checkpoint_every == 0 and plan.adjoint_needs_state:
+            # every step recorded: its start state and its record (substep
+            # states and kept transforms); the backward recomputes nothing
+            n = plan.record_elems()
+            recs = []
+            for _ in range(nsteps):
+                st = u.clone()
+                rec = torch.empty(n, dtype=torch.complex128, device=u.device)
+                plan.step_record(u, rec, dPdx)
+                recs.append((st, rec))
+            ctx.plan = plan
+            ctx.recs = recs
+            ctx.dPdx = dPdx
+            return torch.view_as_real(u).clone()
         k = max(1, min(checkpoint_every, max(1, nsteps)))
         chunks = []
         done = 0
@@ -64,6 +82,11 @@ class _Run(torch.autograd.Function):
     def backward(ctx, gr: torch.Tensor):  # type: ignore[override]
         plan: StepPlan = ctx.plan
         g = torch.view_as_complex(gr.contiguous()).clone()
+        if ctx.recs is not None:
+            for st, rec in reversed(ctx.recs):
+                plan.vjp_recorded(st, rec, g, ctx.dPdx)
+            ctx.recs = None  # the records are consumed
+            return torch.view_as_real(g), None, None, None, None
         for start, count, state in reversed(ctx.chunks):
             if state is None:
                 _vjp(plan, g, count)

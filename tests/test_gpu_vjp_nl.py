@@ -160,3 +160,37 @@ def test_register_z_passes_match_lds_kernels(monkeypatch, shape, kind):
     print(shape, kind, "register vs LDS z passes: step rel %.3e, adjoint rel %.3e" % (es, ea))
     assert es < 1e-13
     assert ea < 1e-13
+
+
+@pytest.mark.parametrize("kind", [KIND_NS, KIND_PERTURBATION])
+@pytest.mark.parametrize("shape", [(24, 65, 16), (24, 33, 48), (24, 32, 96)])
+def test_recorded_steps_gradient_matches_recompute(shape, kind):
+    """checkpoint_every=0: every step recorded by jsd_step_record (substep
+    states and kept transforms) and its adjoint taken by jsd_step_vjp_recorded
+    with nothing recomputed, against the checkpoint-and-recompute gradient
+    (the recorded substep states come from the forward's chained y-derivatives,
+    the recomputed ones from fresh ones: equal to rounding); z lengths 32
+    (LDS z pass), 128 and 256 (the five-row layout)."""
+    g, _ = pair(shape, 2 * np.pi, np.pi)
+    kw = {} if kind == KIND_NS else {"linearise": False}
+    dpdx = -2.0 / 3000.0 if kind == KIND_NS else 0.0
+    plan = StepPlan(g, 3000.0, 0.005, kind, adjoint=True, **kw)
+    u0 = (ns_field(g, seed=4) if kind == KIND_NS else perturbation_field(g, seed=27, energy=1e-2)).to(DEV)
+    J0, g0 = gain_and_grad(u0, plan, 5, checkpoint_every=2, dPdx=dpdx)
+    J1, g1 = gain_and_grad(u0, plan, 5, checkpoint_every=0, dPdx=dpdx)
+    err = rel(g1, g0)
+    print(shape, kind, "recorded vs recompute: J %.3e, grad rel %.3e" % (abs(J1 - J0) / abs(J0), err))
+    assert abs(J1 - J0) <= 1e-13 * abs(J0)
+    assert err < 1e-11
+
+
+def test_step_record_rejects_linearised_plans_and_short_records():
+    g, _ = pair((24, 33, 16), 2 * np.pi, np.pi)
+    lin = StepPlan(g, 3000.0, 0.005, KIND_PERTURBATION, linearise=True, adjoint=True)
+    with pytest.raises(Exception):
+        lin.record_elems()
+    plan = StepPlan(g, 3000.0, 0.005, KIND_NS, adjoint=True)
+    u = ns_field(g, seed=5).to(DEV)
+    short = torch.empty(plan.record_elems() - 1, dtype=torch.complex128, device=DEV)
+    with pytest.raises(Exception):
+        plan.step_record(u, short)
