@@ -1103,6 +1103,14 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
     const int l = l0 + li;
     const size_t line = ((size_t)x * N + (l < N ? l : 0)) * ZT;
     const int ts = zrow_src<T5>(t);
+    // the step's twiddle seeds: with P = 256 (nine row loads) they load
+    // first and arrive with the rows; with P = 128 (sixteen row loads) the
+    // registers are not there, and they load after the DFT
+    double2 w1, w8;
+    if constexpr (P == 256) {
+      w1 = cconj(tw[n1]);
+      w8 = cconj(tw[8 * n1]);
+    }
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) {
       const int n = n1 + N1 * n2;
@@ -1125,7 +1133,10 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
     // powers, restarted from the table at k2 = 8: two loads and two live
     // twiddles instead of fifteen (the powers agree with the table to ~8 ulp)
     {
-      const double2 w1 = cconj(tw[n1]), w8 = cconj(tw[8 * n1]);
+      if constexpr (P != 256) {
+        w1 = cconj(tw[n1]);
+        w8 = cconj(tw[8 * n1]);
+      }
       lds[li * REG + (t * N1 + n1) * Z16_LD] = v[0];
       double2 wk = w1;
 #pragma unroll
