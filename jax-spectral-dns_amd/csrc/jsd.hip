@@ -1071,8 +1071,11 @@ __global__ __launch_bounds__((z256_threads<NT, LN>()), (P == 128 ? 4 : 3)) void 
   // per line: [t][n1][k2] (padded) for the exchanges, [t][z] for the products
   constexpr int REG = NT * N1 * Z16_LD > P * NT ? NT * N1 * Z16_LD : P * NT;
   __shared__ double2 lds[LN * REG];
-  const int x = blockIdx.x;
-  const int l0 = blockIdx.y * LN;  // lines l0 .. l0+LN-1; those >= N load zeros and store nothing
+  // grid (P'x, line groups) taken in dispatch order with the line fastest:
+  // consecutive workgroups read neighbouring lines of one x (2 KB apart)
+  const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+  const int x = bid / gridDim.y;
+  const int l0 = (bid % gridDim.y) * LN;  // lines l0 .. l0+LN-1; those >= N load zeros and store nothing
   const int KZ = Z / 2;
   const int nneg = Z - KZ - 1;  // stored negative bins
   // lanes of the forward transforms (below): FL per packed row, line fli,
