@@ -1,16 +1,16 @@
 """Writes the golden fixtures under tests/golden/ by running the oracle (the
 CPU restatement of the reference step).  Run from the repository root:
 
-    python tests/golden/make_golden.py cfg0
+    python tests/golden/make_golden.py linearised
 
-cfg0: linearised perturbation about U = 1 - y^2, Re = 3000, Lx = Lz = 2 pi,
+linearised: linearised perturbation about U = 1 - y^2, Re = 3000, Lx = Lz = 2 pi,
 24 x 65 x 4 requested modes (25 x 65 x 5 stored), dt = 0.01, 1500 steps;
 initial field: seeded generator, seed 0, energy 1e-6.  Stores the initial
 state, the states after 1, 10 and 1500 steps and the energy history.
 
-    python tests/golden/make_golden.py cfg1
+    python tests/golden/make_golden.py ns
 
-cfg1: see cfg1() (200 nonlinear steps at 128 x 129 x 128; an hour of CPU).
+ns: see ns_128x129x128() (200 nonlinear steps at 128 x 129 x 128; an hour of CPU).
 """
 
 import os
@@ -27,7 +27,7 @@ from jax_spectral_dns_amd import initial  # noqa: E402
 from oracle import jsd_oracle as O  # noqa: E402
 
 
-def cfg0() -> None:
+def linearised_25x65x5() -> None:
     torch.set_num_threads(1)
     shape, re, dt, n = (24, 65, 4), 3000.0, 0.01, O.number_of_steps(15.0, 0.01)
     g = DM.make_grid(shape)
@@ -43,7 +43,7 @@ def cfg0() -> None:
         if t + 1 in (1, 10):
             keep["u%d" % (t + 1)] = u.numpy().copy()
     np.savez_compressed(
-        os.path.join(HERE, "cfg0_linearised_1500.npz"),
+        os.path.join(HERE, "linearised_25x65x5_1500.npz"),
         u0=u0.numpy(),
         u1=keep["u1"],
         u10=keep["u10"],
@@ -51,11 +51,11 @@ def cfg0() -> None:
         energy=np.array(energy),
         params=np.array([re, dt, n, 1e-6, 0.0]),
     )
-    print("cfg0: n=%d gain E(T)/E(0) = %.12f" % (n, energy[-1] / energy[0]))
+    print("linearised: n=%d gain E(T)/E(0) = %.12f" % (n, energy[-1] / energy[0]))
 
 
-def cfg1() -> None:
-    """cfg1: plain nonlinear NS, centreline Re = 5000, Lx = 4 pi, Lz = 2 pi,
+def ns_128x129x128() -> None:
+    """Plain nonlinear NS, centreline Re = 5000, Lx = 4 pi, Lz = 2 pi,
     128 x 129 x 128 requested modes (129 x 129 x 129 stored), dt = 0.005,
     dPdx = -2/Re, 200 steps; initial field: Poiseuille flow plus the seeded
     perturbation (seed 0) at 5 % of its energy.  Stores the energy after every
@@ -77,9 +77,9 @@ def cfg1() -> None:
         energy.append(float(O.energy(d, u)))
         if t + 1 in (1, 10, 100, 200):
             lines["mean%d" % (t + 1)] = torch.stack([u[0, 0, :, 0], u[2, 0, :, 0]]).numpy().copy()
-        print("cfg1 step %d energy %.16e" % (t + 1, energy[-1]), flush=True)
+        print("ns step %d energy %.16e" % (t + 1, energy[-1]), flush=True)
     np.savez_compressed(
-        os.path.join(HERE, "cfg1_ns_200_energy.npz"),
+        os.path.join(HERE, "ns_128x129x128_200_energy.npz"),
         energy=np.array(energy),
         params=np.array([re, dt, n, 0.05, 0.0]),
         **lines,
@@ -87,5 +87,6 @@ def cfg1() -> None:
 
 
 if __name__ == "__main__":
-    for name in sys.argv[1:] or ["cfg0"]:
-        globals()[name]()
+    fixtures = {"linearised": linearised_25x65x5, "ns": ns_128x129x128}
+    for name in sys.argv[1:] or ["linearised"]:
+        fixtures[name]()

@@ -189,7 +189,7 @@ def test_library_rejects_repeated_plans_in_a_batch():
         assert rc == -1 and not h.value
 
 
-def test_cfg3_seeded_64_run_batch():
+def test_seeded_64_uneven_run_batch():
     """The batch workload itself: the 64 runs drawn from seed 0 (Ny up to 129,
     (Nx, Nz) up to (48, 32)), 20 batched steps against the same runs stepped
     one at a time (<= 1e-12), and one Ny = 129, 48 x 32 run of the draw
@@ -233,7 +233,7 @@ def test_cfg3_seeded_64_run_batch():
     batch.close()
 
 
-def _cfg4_reduced(n, adjoint=True):
+def _nonlinear_batch_reduced(n, adjoint=True):
     """Reduced copies of the nonlinear gradient workload: 24 x 65 x 16
     requested modes, Re 3000, Lx 2 pi, Lz pi, dt 0.005, initial energies
     log-spaced in [1e-6, 1e-2] (seeded fields), one plan per run."""
@@ -250,7 +250,7 @@ def _cfg4_reduced(n, adjoint=True):
 def test_nonlinear_batch_steps_equal_individual_steps():
     """A batch of nonlinear perturbation runs steps them concurrently, one
     stream each; every run is bit-identical to stepping its plan alone."""
-    _, plans, u0 = _cfg4_reduced(8, adjoint=False)
+    _, plans, u0 = _nonlinear_batch_reduced(8, adjoint=False)
     ind = [u.clone() for u in u0]
     for p, u in zip(plans, ind):
         p.step(u, 0.0, 6)
@@ -269,7 +269,7 @@ def test_nonlinear_batch_gradient_matches_single_runs_and_oracle():
     """The batched gain gradient of 8 nonlinear runs (checkpoint every 4 of 10
     steps, recompute) equals each run's own discrete adjoint, and run 0 and
     run 7 match autograd of the oracle (the stand-in for jax.grad) <= 1e-9."""
-    g, plans, u0 = _cfg4_reduced(8)
+    g, plans, u0 = _nonlinear_batch_reduced(8)
     n = 10
     J, G = batch_gain_and_grad(plans, u0, n, checkpoint_every=4)
     for r in range(8):

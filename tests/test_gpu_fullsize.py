@@ -1,10 +1,11 @@
 """Full-size and long-horizon checks of the HIP path.
 
-* cfg0 (linearised perturbation, 25 x 65 x 5 stored modes, Re 3000,
-  dt 0.01): 1500 steps against the committed oracle fixture, <= 1e-8
-  relative L2 at T = 15 and <= 1e-11 after one step (BASELINE tolerances).
-* cfg1 (plain nonlinear NS, 129 x 129 x 129 stored modes, Re 5000,
-  dt 0.005): ten consecutive steps against the oracle (<= 1e-11 per step),
+* linearised perturbation, 25 x 65 x 5 stored modes, Re 3000,
+  dt 0.01: 1500 steps against the committed oracle fixture, <= 1e-8
+  relative L2 at T = 15 and <= 1e-11 after one step (the FP64 parity
+  tolerances of the step).
+* plain nonlinear NS, 129 x 129 x 129 stored modes, Re 5000,
+  dt 0.005: ten consecutive steps against the oracle (<= 1e-11 per step),
   200 steps against the oracle's energy history, then size-independent
   properties over several steps: continuity i kx u + D v + i kz w = 0, the
   no-slip / no-penetration wall values, v' = 0 at the walls, Hermitian
@@ -28,8 +29,8 @@ DEV = "cuda"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def test_cfg0_1500_steps_against_fixture():
-    fx = np.load(os.path.join(HERE, "golden", "cfg0_linearised_1500.npz"))
+def test_linearised_25x65x5_1500_steps_against_fixture():
+    fx = np.load(os.path.join(HERE, "golden", "linearised_25x65x5_1500.npz"))
     re, dt, n = fx["params"][:3]
     g = DM.make_grid((24, 65, 4))
     plan = StepPlan(g, re, dt, KIND_PERTURBATION, linearise=True)
@@ -46,7 +47,7 @@ def test_cfg0_1500_steps_against_fixture():
     eT = rel(u, torch.from_numpy(fx["uT"]))
     gain = plan.energy(u) / plan.energy(u0.to(DEV))
     gain_ref = fx["energy"][-1] / fx["energy"][0]
-    print("cfg0 rel L2: 1 step %.2e, 10 steps %.2e, 1500 steps %.2e; gain %.12f vs %.12f" % (e1, e10, eT, gain, gain_ref))
+    print("linearised 25x65x5 rel L2: 1 step %.2e, 10 steps %.2e, 1500 steps %.2e; gain %.12f vs %.12f" % (e1, e10, eT, gain, gain_ref))
     assert e1 < 1e-11
     assert e10 < 1e-10
     assert eT < 1e-8
@@ -54,20 +55,20 @@ def test_cfg0_1500_steps_against_fixture():
     assert abs(energies[0] - fx["energy"][10]) / fx["energy"][10] < 1e-10
 
 
-def _cfg1():
+def _ns_128x129x128():
     g = DM.make_grid((128, 129, 128), 4 * np.pi, 2 * np.pi)
     base = initial.laminar(g)
     u0 = base + initial.perturbation(g, seed=0, energy=0.05 * initial.spectral_energy(g, base))
     return g, torch.from_numpy(u0)
 
 
-def test_cfg1_ten_steps_against_oracle():
+def test_ns_128x129x128_ten_steps_against_oracle():
     """Ten consecutive steps of the full-size workload, each against the
     oracle advanced from the same start: <= 1e-11 (n + 1) relative L2."""
     from oracle import jsd_oracle as O
 
     torch.set_num_threads(min(16, len(os.sched_getaffinity(0))))
-    g, u0 = _cfg1()
+    g, u0 = _ns_128x129x128()
     re, dt = 5000.0, 0.005
     plan = StepPlan(g, re, dt, KIND_NS)
     u = u0.to(DEV)
@@ -78,18 +79,18 @@ def test_cfg1_ten_steps_against_oracle():
         plan.step(u, -2.0 / re, 1)
         ref = O.rk_step(sp, ref, n)
         err = rel(u, ref)
-        print("cfg1 step %d rel L2 %.3e" % (n + 1, err), flush=True)
+        print("ns 128x129x128 step %d rel L2 %.3e" % (n + 1, err), flush=True)
         assert err < 1e-11 * (n + 1)
 
 
-def test_cfg1_200_steps_against_oracle_energy_history():
-    """The full cfg1 horizon: 200 steps of the GPU step against the oracle's
+def test_ns_128x129x128_200_steps_against_oracle_energy_history():
+    """The full 128x129x128 horizon: 200 steps of the GPU step against the oracle's
     energy after every step and its (0,0) mean-flow lines of u and w after
-    1, 10, 100 and 200 steps (tests/golden/make_golden.py cfg1)."""
-    path = os.path.join(HERE, "golden", "cfg1_ns_200_energy.npz")
+    1, 10, 100 and 200 steps (tests/golden/make_golden.py ns)."""
+    path = os.path.join(HERE, "golden", "ns_128x129x128_200_energy.npz")
     fx = np.load(path)
     re, dt, n = fx["params"][:3]
-    g, u0 = _cfg1()
+    g, u0 = _ns_128x129x128()
     plan = StepPlan(g, re, dt, KIND_NS)
     u = u0.to(DEV)
     e0 = plan.energy(u)
@@ -105,13 +106,13 @@ def test_cfg1_200_steps_against_oracle_energy_history():
             ref = torch.from_numpy(fx["mean%d" % (t + 1)])
             mean = torch.stack([u[0, 0, :, 0], u[2, 0, :, 0]]).cpu()
             merr = rel(mean, ref)
-            print("cfg1 step %d: energy rel %.2e, mean flow rel %.2e" % (t + 1, err, merr))
+            print("ns 128x129x128 step %d: energy rel %.2e, mean flow rel %.2e" % (t + 1, err, merr))
             assert merr < 1e-11
-    print("cfg1 worst energy rel over 200 steps %.2e" % worst)
+    print("ns 128x129x128 worst energy rel over 200 steps %.2e" % worst)
 
 
-def test_cfg1_properties_over_steps():
-    g, u0 = _cfg1()
+def test_ns_128x129x128_properties_over_steps():
+    g, u0 = _ns_128x129x128()
     re, dt = 5000.0, 0.005
     plan = StepPlan(g, re, dt, KIND_NS)
     u = u0.to(DEV)

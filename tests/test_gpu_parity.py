@@ -1,6 +1,7 @@
 """Parity of the HIP path (through the C ABI) with the oracle, stage by stage
 and step by step, on seeded inputs.  Tolerances: <= 1e-11 relative L2 per
-step (BASELINE), nonlinear terms and derivatives <= 1e-12."""
+step (the FP64 parity tolerance of the step), nonlinear terms and
+derivatives <= 1e-12."""
 
 import numpy as np
 import pytest

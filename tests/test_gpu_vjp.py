@@ -106,7 +106,7 @@ def test_gain_gradient_against_finite_differences():
     assert worst < 1e-8
 
 
-def test_cfg2_full_horizon_gradient_against_finite_differences():
+def test_linearised_1500_step_gradient_against_finite_differences():
     """The differentiation workload at full length: 48 x 97 x 32, Re 3000,
     dt 0.01, T = 15 (1500 steps), checkpoint every 50; the exact-quadratic
     centred differences along 16 seeded directions (<= 1e-8)."""

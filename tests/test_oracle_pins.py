@@ -391,15 +391,15 @@ def test_parameterisation_pins():
 
 def test_golden_fixtures_match_their_generators():
     """The committed oracle fixtures start from the initial conditions that
-    tests/golden/make_golden.py names: cfg0's stored u0 is the seeded field and
-    one oracle step from it reproduces the stored u1 exactly; cfg1's stored
+    tests/golden/make_golden.py names: the linearised fixture's stored u0 is the seeded field and
+    one oracle step from it reproduces the stored u1 exactly; the NS fixture's stored
     energy history starts at the energy of its seeded initial field, has one
     entry per step and stays finite and positive."""
     from jax_spectral_dns_amd import domain_math as DM
     from jax_spectral_dns_amd import initial
 
     here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-    fx0 = np.load(os.path.join(here, "cfg0_linearised_1500.npz"))
+    fx0 = np.load(os.path.join(here, "linearised_25x65x5_1500.npz"))
     re, dt, n = fx0["params"][:3]
     g0 = DM.make_grid((24, 65, 4))
     assert np.array_equal(fx0["u0"], initial.perturbation(g0, seed=0, energy=1e-6))
@@ -409,7 +409,7 @@ def test_golden_fixtures_match_their_generators():
     assert np.array_equal(u1.numpy(), fx0["u1"])
     assert len(fx0["energy"]) == int(n) + 1
 
-    fx1 = np.load(os.path.join(here, "cfg1_ns_200_energy.npz"))
+    fx1 = np.load(os.path.join(here, "ns_128x129x128_200_energy.npz"))
     g1 = DM.make_grid((128, 129, 128), 4 * np.pi, 2 * np.pi)
     base = initial.laminar(g1)
     u0 = base + initial.perturbation(g1, seed=0, energy=0.05 * initial.spectral_energy(g1, base))
